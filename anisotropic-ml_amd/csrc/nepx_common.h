@@ -1,0 +1,75 @@
+// Shared host/device helpers for libnepx (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdarg.h>
+
+#include "../../include/nepx.h"
+
+namespace nepx {
+
+// ---- error reporting (host) ------------------------------------------------
+void set_error(const char* fmt, ...);
+
+#define NEPX_HIP_CHECK(expr)                                                      \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            ::nepx::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                              __FILE__, __LINE__);                                \
+            return NEPX_EHIP;                                                     \
+        }                                                                         \
+    } while (0)
+
+#define NEPX_LAUNCH_CHECK() NEPX_HIP_CHECK(hipGetLastError())
+
+#define NEPX_REQUIRE(cond, code, ...)      \
+    do {                                   \
+        if (!(cond)) {                     \
+            ::nepx::set_error(__VA_ARGS__); \
+            return (code);                 \
+        }                                  \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---- compile-time helpers --------------------------------------------------
+constexpr double csqrt(double x) {
+    double r = x > 1.0 ? x : 1.0;
+    for (int i = 0; i < 64; ++i) r = 0.5 * (r + x / r);
+    return r;
+}
+constexpr double cfact(int n) { return n <= 1 ? 1.0 : n * cfact(n - 1); }
+
+// ---- quaternion helpers (float, device) ------------------------------------
+// R(q) v = (w^2-|u|^2) v + 2 w (u x v) + 2 (u.v) u  as a 3x3 matrix, written
+// out so that it equals the reference formula (PosToPtsKernels.py:54-71) for
+// any q, unit or not.
+struct Mat3 {
+    float m[9];
+};
+
+__device__ __forceinline__ Mat3 quat_matrix(float w, float x, float y, float z) {
+    const float a = w * w - (x * x + y * y + z * z);
+    Mat3 R;
+    R.m[0] = a + 2.f * x * x;
+    R.m[1] = 2.f * (x * y - w * z);
+    R.m[2] = 2.f * (x * z + w * y);
+    R.m[3] = 2.f * (x * y + w * z);
+    R.m[4] = a + 2.f * y * y;
+    R.m[5] = 2.f * (y * z - w * x);
+    R.m[6] = 2.f * (x * z - w * y);
+    R.m[7] = 2.f * (y * z + w * x);
+    R.m[8] = a + 2.f * z * z;
+    return R;
+}
+
+__device__ __forceinline__ void mat_vec(const Mat3& R, float bx, float by, float bz, float& ox,
+                                        float& oy, float& oz) {
+    ox = R.m[0] * bx + R.m[1] * by + R.m[2] * bz;
+    oy = R.m[3] * bx + R.m[4] * by + R.m[5] * bz;
+    oz = R.m[6] * bx + R.m[7] * by + R.m[8] * bz;
+}
+
+}  // namespace nepx

@@ -1,0 +1,154 @@
+// Energy network on MFMA: forward and backward-to-input for tiles of 16 pairs.
+//
+// TheEnergyNet (nep_simulator/ForceEvaluator.py:6-26): D -> H -> (H)*NH -> 1 with an
+// activation after every layer but the last.  Each wave computes the
+// transposed products  Z^T (H x 16) = W (H x K) . X^T (K x 16)  with
+// v_mfma_f32_16x16x4_f32, so hidden units run along M and the 16 pairs along N.
+// In the 16x16 accumulator a lane holds pair (lane % 16) and units
+// 16*mt + 4*(lane/16) + r, r = 0..3.  The K order of every product is
+// permuted so that k-step s takes unit 16*(s/4) + 4*(lane/16) + s%4 from each
+// lane: the accumulator of one layer is then the B operand of the next with
+// no cross-lane movement.  The host packs every A operand in that order
+// (nepx_net_pack), one coalesced 256-byte read per MFMA.
+#pragma once
+#include "nepx_common.h"
+
+namespace nepx {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int round16(int v) { return (v + 15) / 16 * 16; }
+
+// Offsets (in floats) of the packed operands; identical on host and device.
+struct NetLayout {
+    int DP, HP, NL;  // padded input, padded width, layers that produce hidden units
+    __host__ __device__ constexpr int fwd_size(int k) const {  // layer k = 0..NL-1
+        return (HP / 16) * ((k == 0 ? DP : HP) / 4) * 64;
+    }
+    __host__ __device__ constexpr int bwd_size(int k) const {
+        return ((k == 0 ? DP : HP) / 16) * (HP / 4) * 64;
+    }
+    __host__ __device__ constexpr int fwd_off(int k) const {
+        return k == 0 ? 0 : fwd_off(k - 1) + fwd_size(k - 1);
+    }
+    __host__ __device__ constexpr int bwd_off(int k) const {
+        return k == 0 ? fwd_off(NL) : bwd_off(k - 1) + bwd_size(k - 1);
+    }
+    __host__ __device__ constexpr int bias_off(int k) const { return bwd_off(NL) + k * HP; }
+    __host__ __device__ constexpr int wout_off() const { return bias_off(NL); }
+    __host__ __device__ constexpr int bout_off() const { return wout_off() + HP; }
+    __host__ __device__ constexpr int total() const { return bout_off() + 4; }
+};
+
+template <int DP_, int HP_, int NL_>
+struct NetCfg {
+    static constexpr int DP = DP_, HP = HP_, NL = NL_;
+    static constexpr int MT = HP / 16;   // M tiles of a hidden layer
+    static constexpr int KD = DP / 4;    // k-steps over the input
+    static constexpr int KH = HP / 4;    // k-steps over a hidden layer
+    static constexpr int MD = DP / 16;   // M tiles of dE/dG
+    static constexpr NetLayout layout() { return NetLayout{DP, HP, NL}; }
+};
+
+__device__ __forceinline__ float act_fwd(float z, int act) {
+    return act == NEPX_ACT_TANH ? tanhf(z) : fmaxf(z, 0.f);
+}
+__device__ __forceinline__ float act_der(float h, int act) {  // in terms of the output
+    return act == NEPX_ACT_TANH ? 1.f - h * h : (h > 0.f ? 1.f : 0.f);
+}
+
+// One tile of 16 pairs.  gin: lane's 4-float chunks of its pair's input
+// (DP/16 chunks, chunk c holds inputs 16c + 4*(lane/16) + 0..3).
+// Outputs: y (network output, valid in every lane for pair lane%16) and
+// dgo (dE/dG chunks in the same layout), scaled by `scale`.
+template <class N>
+__device__ __forceinline__ float ffnn_tile(const float* __restrict__ W, int act,
+                                           const v4f (&gin)[N::MD], float scale,
+                                           v4f (&dgo)[N::MD]) {
+    constexpr NetLayout Lay = N::layout();
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+    v4f h[N::NL][N::MT];  // activations of every hidden layer (D layout)
+    // ---- forward ----
+#pragma unroll
+    for (int k = 0; k < N::NL; ++k) {
+        const float* Wf = W + Lay.fwd_off(k);
+        const float* bk = W + Lay.bias_off(k);
+        const int ks = (k == 0) ? N::KD : N::KH;
+        v4f acc[N::MT];
+#pragma unroll
+        for (int mt = 0; mt < N::MT; ++mt) {
+            const float4 b4 = *reinterpret_cast<const float4*>(bk + 16 * mt + 4 * g);
+            acc[mt] = v4f{b4.x, b4.y, b4.z, b4.w};
+        }
+#pragma unroll
+        for (int s = 0; s < ks; ++s) {
+            const float bop = (k == 0) ? gin[s >> 2][s & 3] : h[k - 1][s >> 2][s & 3];
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt) {
+                const float aop = Wf[(mt * ks + s) * 64 + lane];
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, acc[mt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < N::MT; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) h[k][mt][r] = act_fwd(acc[mt][r], act);
+    }
+    // ---- output layer (dot over units, reduced across the 4 lane groups) ----
+    const float* wo = W + Lay.wout_off();
+    float y = 0.f;
+    v4f delta[N::MT];
+#pragma unroll
+    for (int mt = 0; mt < N::MT; ++mt) {
+        const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * mt + 4 * g);
+        const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            y += wv[r] * h[N::NL - 1][mt][r];
+            delta[mt][r] = scale * wv[r] * act_der(h[N::NL - 1][mt][r], act);
+        }
+    }
+    y += __shfl_xor(y, 16);
+    y += __shfl_xor(y, 32);
+    y += W[Lay.bout_off()];
+    // ---- backward through the hidden layers ----
+#pragma unroll
+    for (int k = N::NL - 1; k >= 1; --k) {
+        const float* Wb = W + Lay.bwd_off(k);
+        v4f acc[N::MT];
+#pragma unroll
+        for (int mt = 0; mt < N::MT; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < N::KH; ++s) {
+            const float bop = delta[s >> 2][s & 3];
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt) {
+                const float aop = Wb[(mt * N::KH + s) * 64 + lane];
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, acc[mt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < N::MT; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) delta[mt][r] = acc[mt][r] * act_der(h[k - 1][mt][r], act);
+    }
+    // ---- dE/dG = W_1^T delta_1 ----
+    {
+        const float* Wb = W + Lay.bwd_off(0);
+#pragma unroll
+        for (int mt = 0; mt < N::MD; ++mt) dgo[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < N::KH; ++s) {
+            const float bop = delta[s >> 2][s & 3];
+#pragma unroll
+            for (int mt = 0; mt < N::MD; ++mt) {
+                const float aop = Wb[(mt * N::KH + s) * 64 + lane];
+                dgo[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, dgo[mt], 0, 0, 0);
+            }
+        }
+    }
+    return y;
+}
+
+}  // namespace nepx

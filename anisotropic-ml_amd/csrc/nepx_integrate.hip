@@ -1,0 +1,243 @@
+// FP64 rigid-body NVT integrator (two-step Nose-Hoover translation + NO_SQUISH
+// rotation), per-system kinetic-energy reduction and thermostat update.
+//
+// Per step, after F and tau are known (nep_simulator/NVT.py:17-129):
+//   v <- v*theta + dt/2 F/m ; dp = dt q (x) (0, R(q*) tau) ; p <- p e_r + dp   (step two)
+//   v <- (v + dt/2 F/m) theta ; x <- wrap(x + v dt)                          (step one)
+//   p <- (p + dp) e_r ; free rotor 3(dt/2) 2(dt/2) 1(dt) 2(dt/2) 3(dt/2) ; q <- q/|q|
+// followed by K_t = sum m v^2/2, K_r = sum_k s_k^2/(2 I_k), s = q* (x) p / 2, and
+// the thermostat update that the reference performs at the start of the next
+// step (IntegrateStepTwoKernels.py:93-125), done here once per system at the
+// tail of the step from the same inputs.
+#include "nepx_common.h"
+
+namespace nepx {
+
+constexpr int kIB = 256;  // particles per integrator block (system ranges are block aligned)
+
+struct Q4 {
+    double w, x, y, z;
+};
+
+__device__ __forceinline__ Q4 qmul(const Q4& a, const Q4& b) {
+    return Q4{a.w * b.w - (a.x * b.x + a.y * b.y + a.z * b.z),
+              a.w * b.x + b.w * a.x + (a.y * b.z - a.z * b.y),
+              a.w * b.y + b.w * a.y + (a.z * b.x - a.x * b.z),
+              a.w * b.z + b.w * a.z + (a.x * b.y - a.y * b.x)};
+}
+
+// P_k v of the free-rotor sub-steps (IntegrateStepOneKernels.py:45-157)
+__device__ __forceinline__ Q4 perm(int k, const Q4& v) {
+    if (k == 1) return Q4{-v.x, v.w, v.z, -v.y};
+    if (k == 2) return Q4{-v.y, -v.z, v.w, v.x};
+    return Q4{-v.z, v.y, -v.x, v.w};
+}
+
+__device__ __forceinline__ void rotor(int k, double h, double Ik, Q4& p, Q4& q) {
+    const Q4 pp = perm(k, p), qq = perm(k, q);
+    const double phi = (0.25 / Ik) * (p.w * qq.w + p.x * qq.x + p.y * qq.y + p.z * qq.z);
+    double s, c;
+    sincos(h * phi, &s, &c);
+    p = Q4{c * p.w + s * pp.w, c * p.x + s * pp.x, c * p.y + s * pp.y, c * p.z + s * pp.z};
+    q = Q4{c * q.w + s * qq.w, c * q.x + s * qq.x, c * q.y + s * qq.y, c * q.z + s * qq.z};
+}
+
+__device__ __forceinline__ double rot_ke(const Q4& q, const Q4& p, const double* moi) {
+    const Q4 s = qmul(Q4{q.w, -q.x, -q.y, -q.z}, p);
+    const double sx = 0.5 * s.x, sy = 0.5 * s.y, sz = 0.5 * s.z;
+    return 0.5 * (sx * sx / moi[0] + sy * sy / moi[1] + sz * sz / moi[2]);
+}
+
+__device__ void block_reduce4(double a, double b, double c, double dmax, double* out) {
+    __shared__ double red[4][kIB];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    red[3][threadIdx.x] = dmax;
+    __syncthreads();
+    for (int off = kIB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+            red[2][threadIdx.x] += red[2][threadIdx.x + off];
+            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+        out[2] = red[2][0];
+        out[3] = red[3][0];
+    }
+}
+
+__device__ __forceinline__ double disp2(const nepx_state& st, int i, double L) {
+    double dx = st.x[i] - st.xb[i], dy = st.y[i] - st.yb[i], dz = st.z[i] - st.zb[i];
+    dx -= L * rint(dx / L);
+    dy -= L * rint(dy / L);
+    dz -= L * rint(dz / L);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// mode 0: full step; mode 1: kinetic energies only (initialisation).
+__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
+    const int i = blockIdx.x * kIB + threadIdx.x;
+    double kt = 0.0, kr = 0.0, u = 0.0, d2 = 0.0;
+    if (i < st.n && st.sys[i] >= 0) {
+        const int sidx = st.sys[i];
+        const nepx_sysparam& sp = st.sp[sidx];
+        const nepx_thermo& th = st.th[sidx];
+        const double dt = sp.dt, L = sp.L;
+        const double m = st.mass[i];
+        double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
+        Q4 q{st.q0[i], st.q1[i], st.q2[i], st.q3[i]};
+        Q4 p{st.p0[i], st.p1[i], st.p2[i], st.p3[i]};
+        if (mode == 0) {
+            const double theta = th.theta, er = th.e_r;
+            const double ax = st.fx[i] / m, ay = st.fy[i] / m, az = st.fz[i] / m;
+            // step two: translation and torque half kicks
+            vx = vx * theta + 0.5 * dt * ax;
+            vy = vy * theta + 0.5 * dt * ay;
+            vz = vz * theta + 0.5 * dt * az;
+            const Q4 qc{q.w, -q.x, -q.y, -q.z};
+            const Q4 tb = qmul(qmul(qc, Q4{0.0, st.tx[i], st.ty[i], st.tz[i]}), q);  // R(q*) tau
+            const Q4 dq = qmul(q, Q4{0.0, tb.x, tb.y, tb.z});
+            const Q4 dp{dt * dq.w, dt * dq.x, dt * dq.y, dt * dq.z};
+            p = Q4{p.w * er + dp.w, p.x * er + dp.x, p.y * er + dp.y, p.z * er + dp.z};
+            // step one: translation
+            vx = (vx + 0.5 * dt * ax) * theta;
+            vy = (vy + 0.5 * dt * ay) * theta;
+            vz = (vz + 0.5 * dt * az) * theta;
+            double x = st.x[i] + vx * dt, y = st.y[i] + vy * dt, z = st.z[i] + vz * dt;
+            x = x > 0.5 * L ? x - L : x;
+            x = x < -0.5 * L ? x + L : x;
+            y = y > 0.5 * L ? y - L : y;
+            y = y < -0.5 * L ? y + L : y;
+            z = z > 0.5 * L ? z - L : z;
+            z = z < -0.5 * L ? z + L : z;
+            st.x[i] = x;
+            st.y[i] = y;
+            st.z[i] = z;
+            st.vx[i] = vx;
+            st.vy[i] = vy;
+            st.vz[i] = vz;
+            // step one: rotation
+            p = Q4{(p.w + dp.w) * er, (p.x + dp.x) * er, (p.y + dp.y) * er, (p.z + dp.z) * er};
+            rotor(3, 0.5 * dt, sp.moi[2], p, q);
+            rotor(2, 0.5 * dt, sp.moi[1], p, q);
+            rotor(1, dt, sp.moi[0], p, q);
+            rotor(2, 0.5 * dt, sp.moi[1], p, q);
+            rotor(3, 0.5 * dt, sp.moi[2], p, q);
+            const double nq = sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+            q = Q4{q.w / nq, q.x / nq, q.y / nq, q.z / nq};
+            st.q0[i] = q.w;
+            st.q1[i] = q.x;
+            st.q2[i] = q.y;
+            st.q3[i] = q.z;
+            st.p0[i] = p.w;
+            st.p1[i] = p.x;
+            st.p2[i] = p.y;
+            st.p3[i] = p.z;
+            d2 = disp2(st, i, L);
+        }
+        reinterpret_cast<float4*>(st.qf)[i] = make_float4((float)q.w, (float)q.x, (float)q.y, (float)q.z);
+        kt = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        kr = rot_ke(q, p, sp.moi);
+        u = st.upot ? st.upot[i] : 0.0;
+    }
+    block_reduce4(kt, kr, u, d2, st.partials + 4 * (size_t)blockIdx.x);
+}
+
+// One block per system: fixed-order sums of the block partials, thermostat
+// update, list rebuild test.
+__global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* flags, int update) {
+    const int s = blockIdx.x;
+    const nepx_sysparam& sp = st.sp[s];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int k = sp.blk_start + threadIdx.x; k < sp.blk_end; k += kIB) {
+        const double* pp = st.partials + 4 * (size_t)k;
+        a += pp[0];
+        b += pp[1];
+        c += pp[2];
+        d = fmax(d, pp[3]);
+    }
+    __shared__ double out[4];
+    block_reduce4(a, b, c, d, out);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        nepx_thermo& th = st.th[s];
+        const double Kt = out[0], Kr = out[1];
+        th.Kt = Kt;
+        th.Kr = Kr;
+        th.U = out[2];
+        th.max_disp2 = out[3];
+        if (update) {
+            const double dt = sp.dt, tau = sp.tau, kT = sp.kT;
+            const double cfac = 0.5 * ((dt / tau) / tau);
+            const double Tt = 2.0 * Kt / sp.ndof;
+            const double xi_p = th.xi + cfac * (Tt / kT - 1.0);
+            const double xi = xi_p + cfac * (Tt / kT - 1.0);
+            th.eta += xi_p * dt;
+            th.xi = xi;
+            th.theta = exp(-0.5 * xi * dt);
+            const double Tr = 2.0 * Kr / sp.rndof;
+            const double xr_p = th.xi_r + cfac * (Tr / kT - 1.0);
+            const double xr = xr_p + cfac * (Tr / kT - 1.0);
+            th.eta_r += xr_p * dt;
+            th.xi_r = xr;
+            th.e_r = exp(-0.5 * dt * xr);
+        }
+        const double half = 0.5 * sp.skin;
+        if (flags && out[3] > half * half) flags[0] = 1;
+    }
+}
+
+}  // namespace nepx
+
+using namespace nepx;
+
+static int check_nvt(const nepx_state* st) {
+    NEPX_REQUIRE(st && st->n > 0 && st->n_sys > 0, NEPX_EINVAL, "empty state");
+    NEPX_REQUIRE(st->x && st->vx && st->q0 && st->p0 && st->mass && st->fx && st->tx && st->sys && st->sp &&
+                     st->th && st->partials && st->qf && st->xb,
+                 NEPX_EINVAL, "null state buffer");
+    return NEPX_OK;
+}
+
+extern "C" int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream) {
+    int rc = check_nvt(st);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const int nb = (st->n + kIB - 1) / kIB;
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
+extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, void* stream) {
+    int rc = check_nvt(st);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const int nb = (st->n + kIB - 1) / kIB;
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr, 1);
+    NEPX_LAUNCH_CHECK();
+    (void)nl;
+    return NEPX_OK;
+}
+
+extern "C" int nepx_md_step(nepx_state* st, nepx_nlist* nl, const nepx_pair_model* m, const nepx_net* net,
+                            float clip, float* pair_out, void* stream) {
+    int rc = nepx_nlist_build(st, nl, 0, stream);
+    if (rc) return rc;
+    rc = nepx_pairs_filter(st, nl, stream);
+    if (rc) return rc;
+    rc = nepx_pair_fused(m, net, st->qf, nl->pair_i, nl->pair_j, nl->pair_t, nl->flags + 3, nl->cap, clip,
+                         pair_out, stream);
+    if (rc) return rc;
+    rc = nepx_forces_gather(st, nl, pair_out, stream);
+    if (rc) return rc;
+    return nepx_nvt_step(st, nl, stream);
+}

@@ -1,0 +1,546 @@
+// Neighbour list on the GPU: cell binning, half Verlet list within r_list
+// (rows i<j sorted by j, reverse rows sorted by i), per-step COM filter with
+// an FP64 predicate and deterministic compaction, and the deterministic
+// gather of per-pair forces.
+//
+// Replaces the host cKDTree list (nep_simulator/Sim.py:85-92,192-198), the
+// per-step COM filter (nep_simulator/DescriptorGenerator.py:168-185) and the
+// index_add_ scatter (nep_simulator/ForceEvaluator.py:135-142).
+//
+// Pair predicate (all paths, bit-exact with the oracle): t = X_j - X_i per
+// component, one wrap (t > L/2 -> t - L, t < -L/2 -> t + L), then
+// d2 = (tx*tx + ty*ty) + tz*tz in FP64 with no fused multiply-add, d2 < r*r.
+#include "nepx_common.h"
+
+namespace nepx {
+
+constexpr int kNB = 256;        // threads per block
+constexpr int kScanBlock = 1024;
+constexpr int kSortMax = 2048;  // longest row the in-LDS sort accepts
+constexpr int kSlotsPerBlock = 1024;
+
+enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5 };
+
+__device__ __forceinline__ double wrap1(double t, double L) {
+    t = t > 0.5 * L ? t - L : t;
+    t = t < -0.5 * L ? L + t : t;
+    return t;
+}
+
+__device__ __forceinline__ double pair_d2(const nepx_state& st, double L, int i, int j, double& tx,
+                                          double& ty, double& tz) {
+    tx = wrap1(__dsub_rn(st.x[j], st.x[i]), L);
+    ty = wrap1(__dsub_rn(st.y[j], st.y[i]), L);
+    tz = wrap1(__dsub_rn(st.z[j], st.z[i]), L);
+    return __dadd_rn(__dadd_rn(__dmul_rn(tx, tx), __dmul_rn(ty, ty)), __dmul_rn(tz, tz));
+}
+
+__device__ __forceinline__ int cell_coord(double x, double L, int nc) {
+    int c = (int)floor((x + 0.5 * L) / L * (double)nc);
+    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__device__ __forceinline__ bool want_build(const int32_t* flags) { return flags[F_REBUILD] != 0; }
+
+__global__ void k_set_flag(int32_t* flags, int idx, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) flags[idx] = v;
+}
+
+__global__ void k_zero_i32(int32_t* a, int64_t n, const int32_t* flags) {
+    if (flags && !want_build(flags)) return;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        a[k] = 0;
+}
+
+__global__ void k_cells_count(nepx_state st, nepx_nlist nl) {
+    if (!want_build(nl.flags)) return;
+    const int i = blockIdx.x * kNB + threadIdx.x;
+    if (i >= st.n) return;
+    const int s = st.sys[i];
+    if (s < 0) return;
+    const nepx_sysparam& sp = st.sp[s];
+    if (sp.nc <= 0) return;
+    const int cx = cell_coord(st.x[i], sp.L, sp.nc), cy = cell_coord(st.y[i], sp.L, sp.nc),
+              cz = cell_coord(st.z[i], sp.L, sp.nc);
+    const int c = sp.cell_off + (cx * sp.nc + cy) * sp.nc + cz;
+    nl.cell_of[i] = c;
+    atomicAdd(&nl.cell_count[c], 1);
+}
+
+// Exclusive scan of a[0..n) into out[0..n]; out[n] = total.  One block.
+__device__ void block_exclusive_scan(const int32_t* a, int32_t* out, int n) {
+    __shared__ int32_t buf[kScanBlock];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += kScanBlock) {
+        const int k = base + threadIdx.x;
+        const int32_t v = k < n ? a[k] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < kScanBlock; off <<= 1) {
+            int32_t add = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (k < n) out[k] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == kScanBlock - 1) carry += buf[kScanBlock - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_cells(nepx_nlist nl) {
+    if (!want_build(nl.flags)) return;
+    block_exclusive_scan(nl.cell_count, nl.cell_start, nl.n_cells_total);
+}
+
+__global__ void k_cells_fill(nepx_state st, nepx_nlist nl) {
+    if (!want_build(nl.flags)) return;
+    const int i = blockIdx.x * kNB + threadIdx.x;
+    if (i >= st.n) return;
+    const int s = st.sys[i];
+    if (s < 0 || st.sp[s].nc <= 0) return;
+    const int c = nl.cell_of[i];
+    const int pos = nl.cell_start[c] + atomicAdd(&nl.cell_fill[c], 1);
+    nl.cell_list[pos] = i;
+}
+
+// Visit every candidate j != i of particle i (27 cells, or the whole system
+// when the box is too small for three cells a side), calling f(j, d2, r2).
+template <class F>
+__device__ __forceinline__ void for_candidates(const nepx_state& st, const nepx_nlist& nl, int i, F&& f) {
+    const int s = st.sys[i];
+    const nepx_sysparam& sp = st.sp[s];
+    const double L = sp.L;
+    if (sp.nc <= 0) {
+        for (int j = sp.p_start; j < sp.p_end; ++j) {
+            if (j == i || st.sys[j] != s) continue;
+            f(j);
+        }
+        return;
+    }
+    const int nc = sp.nc;
+    const int cx = cell_coord(st.x[i], L, nc), cy = cell_coord(st.y[i], L, nc), cz = cell_coord(st.z[i], L, nc);
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int ax = (cx + dx + nc) % nc;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int ay = (cy + dy + nc) % nc;
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int az = (cz + dz + nc) % nc;
+                const int c = sp.cell_off + (ax * nc + ay) * nc + az;
+                const int b = nl.cell_start[c], e = nl.cell_start[c + 1];
+                for (int k = b; k < e; ++k) {
+                    const int j = nl.cell_list[k];
+                    if (j != i) f(j);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_nbr_count(nepx_state st, nepx_nlist nl) {
+    if (!want_build(nl.flags)) return;
+    const int i = blockIdx.x * kNB + threadIdx.x;
+    if (i >= st.n) return;
+    int nf = 0, nr = 0;
+    if (st.sys[i] >= 0) {
+        const nepx_sysparam& sp = st.sp[st.sys[i]];
+        const double r2 = __dmul_rn(sp.r_list, sp.r_list);
+        for_candidates(st, nl, i, [&](int j) {
+            double tx, ty, tz;
+            if (pair_d2(st, sp.L, i, j, tx, ty, tz) < r2) {
+                if (j > i) ++nf;
+                else ++nr;
+            }
+        });
+    }
+    nl.row_count[i] = nf;
+    nl.rrow_count[i] = nr;
+    if (nf > kSortMax || nr > kSortMax) atomicMax(&nl.flags[F_MAXROW], nf > nr ? nf : nr);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_rows(nepx_nlist nl, int n) {
+    if (!want_build(nl.flags)) return;
+    if (blockIdx.x == 0) {
+        block_exclusive_scan(nl.row_count, nl.row_start, n);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            nl.flags[F_PL] = nl.row_start[n];
+            if ((int64_t)nl.row_start[n] > nl.cap || nl.flags[F_MAXROW] > kSortMax) nl.flags[F_OVERFLOW] = 1;
+        }
+    } else {
+        block_exclusive_scan(nl.rrow_count, nl.rrow_start, n);
+    }
+}
+
+__global__ void k_nbr_fill(nepx_state st, nepx_nlist nl) {
+    if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
+    const int i = blockIdx.x * kNB + threadIdx.x;
+    if (i >= st.n || st.sys[i] < 0) return;
+    const nepx_sysparam& sp = st.sp[st.sys[i]];
+    const double r2 = __dmul_rn(sp.r_list, sp.r_list);
+    int pf = nl.row_start[i], pr = nl.rrow_start[i];
+    for_candidates(st, nl, i, [&](int j) {
+        double tx, ty, tz;
+        if (pair_d2(st, sp.L, i, j, tx, ty, tz) < r2) {
+            if (j > i) {
+                nl.nbr[pf] = j;
+                nl.slot_row[pf] = i;
+                ++pf;
+            } else {
+                nl.rnbr[pr++] = j;
+            }
+        }
+    });
+}
+
+// Sort each row ascending: one block of 256 threads per row, bitonic in LDS.
+__global__ __launch_bounds__(256) void k_sort_rows(nepx_nlist nl, int n, int reverse) {
+    if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
+    __shared__ int32_t buf[kSortMax];
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int32_t* start = reverse ? nl.rrow_start : nl.row_start;
+    int32_t* data = reverse ? nl.rnbr : nl.nbr;
+    const int b = start[i], len = start[i + 1] - b;
+    if (len <= 1) return;
+    int p2 = 1;
+    while (p2 < len) p2 <<= 1;
+    for (int k = threadIdx.x; k < p2; k += 256) buf[k] = k < len ? data[b + k] : 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < p2; k += 256) {
+                const int partner = k ^ stride;
+                if (partner > k) {
+                    const bool up = (k & size) == 0;
+                    const int32_t a = buf[k], c = buf[partner];
+                    if ((a > c) == up) {
+                        buf[k] = c;
+                        buf[partner] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < len; k += 256) data[b + k] = buf[k];
+}
+
+// Reverse entries hold the partner i; replace it by the slot of (i, k) in row i.
+__global__ void k_rev_slots(nepx_nlist nl, int n) {
+    if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
+    const int k = blockIdx.x * kNB + threadIdx.x;
+    if (k >= n) return;
+    for (int e = nl.rrow_start[k]; e < nl.rrow_start[k + 1]; ++e) {
+        const int i = nl.rnbr[e];
+        int lo = nl.row_start[i], hi = nl.row_start[i + 1] - 1, slot = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const int v = nl.nbr[mid];
+            if (v == k) {
+                slot = mid;
+                break;
+            }
+            if (v < k) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        if (slot < 0) nl.flags[F_OVERFLOW] = 2;  // inconsistent list: never expected
+        nl.rnbr[e] = slot;
+    }
+}
+
+__global__ void k_build_done(nepx_state st, nepx_nlist nl) {
+    if (!want_build(nl.flags)) return;
+    const int i = blockIdx.x * kNB + threadIdx.x;
+    if (i < st.n) {
+        st.xb[i] = st.x[i];
+        st.yb[i] = st.y[i];
+        st.zb[i] = st.z[i];
+    }
+}
+
+__global__ void k_build_finish(nepx_nlist nl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && nl.flags[F_REBUILD]) {
+        nl.flags[F_REBUILD] = 0;
+        nl.flags[F_BUILDS] += 1;
+    }
+}
+
+// ---- explicit pair list (API path) ----------------------------------------
+__global__ void k_pairs_count(nepx_nlist nl, const int64_t* pairs, int64_t m, int n) {
+    const int64_t k = blockIdx.x * (int64_t)kNB + threadIdx.x;
+    if (k >= m) return;
+    int a = (int)pairs[2 * k], b = (int)pairs[2 * k + 1];
+    if (a == b || a < 0 || b < 0 || a >= n || b >= n) {
+        nl.flags[F_OVERFLOW] = 3;
+        return;
+    }
+    if (a > b) {
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    atomicAdd(&nl.row_count[a], 1);
+    atomicAdd(&nl.rrow_count[b], 1);
+}
+
+__global__ void k_pairs_fill(nepx_nlist nl, const int64_t* pairs, int64_t m, int32_t* fill_f, int32_t* fill_r) {
+    if (nl.flags[F_OVERFLOW]) return;
+    const int64_t k = blockIdx.x * (int64_t)kNB + threadIdx.x;
+    if (k >= m) return;
+    int a = (int)pairs[2 * k], b = (int)pairs[2 * k + 1];
+    if (a > b) {
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    const int pf = nl.row_start[a] + atomicAdd(&fill_f[a], 1);
+    nl.nbr[pf] = b;
+    nl.slot_row[pf] = a;
+    const int pr = nl.rrow_start[b] + atomicAdd(&fill_r[b], 1);
+    nl.rnbr[pr] = a;
+}
+
+__global__ void k_rows_maxlen(nepx_nlist nl, int n) {
+    const int i = blockIdx.x * kNB + threadIdx.x;
+    if (i >= n) return;
+    const int nf = nl.row_count[i], nr = nl.rrow_count[i];
+    if (nf > kSortMax || nr > kSortMax) atomicMax(&nl.flags[F_MAXROW], nf > nr ? nf : nr);
+}
+
+// ---- per-step filter and compaction ---------------------------------------
+__device__ __forceinline__ bool slot_active(const nepx_state& st, const nepx_nlist& nl, int s, int& i, int& j,
+                                            double& tx, double& ty, double& tz) {
+    i = nl.slot_row[s];
+    j = nl.nbr[s];
+    const nepx_sysparam& sp = st.sp[st.sys[i]];
+    return pair_d2(st, sp.L, i, j, tx, ty, tz) < __dmul_rn(sp.rc, sp.rc);
+}
+
+__global__ __launch_bounds__(kNB) void k_filter_count(nepx_state st, nepx_nlist nl) {
+    const int PL = nl.flags[F_PL];
+    const int base = blockIdx.x * kSlotsPerBlock;
+    if (base >= PL || nl.flags[F_OVERFLOW]) {
+        if (threadIdx.x == 0) nl.blk_count[blockIdx.x] = 0;
+        return;
+    }
+    int cnt = 0;
+    for (int k = 0; k < kSlotsPerBlock / kNB; ++k) {
+        const int s = base + k * kNB + threadIdx.x;
+        int i, j;
+        double tx, ty, tz;
+        if (s < PL && slot_active(st, nl, s, i, j, tx, ty, tz)) ++cnt;
+    }
+    __shared__ int red[kNB];
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = kNB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) nl.blk_count[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_filter_scan(nepx_nlist nl, int nblk) {
+    // blk_count -> exclusive offsets in place via cidx-free scratch (blk_count[nblk] = total)
+    __shared__ int32_t tmp_total;
+    block_exclusive_scan(nl.blk_count, nl.blk_count, nblk);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        tmp_total = nl.blk_count[nblk];
+        nl.flags[F_P] = nl.flags[F_OVERFLOW] ? 0 : tmp_total;
+    }
+}
+
+__global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist nl) {
+    const int PL = nl.flags[F_PL];
+    const int base = blockIdx.x * kSlotsPerBlock;
+    if (base >= PL || nl.flags[F_OVERFLOW]) return;
+    __shared__ int wsum[kNB / 64];
+    int out = nl.blk_count[blockIdx.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = 0; k < kSlotsPerBlock / kNB; ++k) {
+        const int s = base + k * kNB + threadIdx.x;
+        int i = 0, j = 0;
+        double tx = 0, ty = 0, tz = 0;
+        const bool act = s < PL && slot_active(st, nl, s, i, j, tx, ty, tz);
+        const uint64_t mask = __ballot(act);
+        const int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = __popcll(mask);
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wv; ++w) woff += wsum[w];
+        int tot = 0;
+        for (int w = 0; w < kNB / 64; ++w) tot += wsum[w];
+        if (s < PL) {
+            if (act) {
+                const int c = out + woff + before;
+                nl.cidx[s] = c;
+                nl.pair_i[c] = i;
+                nl.pair_j[c] = j;
+                reinterpret_cast<float4*>(nl.pair_t)[c] = make_float4((float)tx, (float)ty, (float)tz, 0.f);
+            } else {
+                nl.cidx[s] = -1;
+            }
+        }
+        out += tot;
+        __syncthreads();
+    }
+}
+
+// ---- gather ----------------------------------------------------------------
+__global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, const float4* __restrict__ out) {
+    const int k = blockIdx.x * kNB + threadIdx.x;
+    if (k >= st.n) return;
+    double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, u = 0;
+    if (st.sys[k] >= 0 && !nl.flags[F_OVERFLOW]) {
+        for (int s = nl.row_start[k]; s < nl.row_start[k + 1]; ++s) {
+            const int c = nl.cidx[s];
+            if (c < 0) continue;
+            const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 1];
+            fx += a.x;
+            fy += a.y;
+            fz += a.z;
+            u += a.w;
+            tx += b.x;
+            ty += b.y;
+            tz += b.z;
+        }
+        for (int e = nl.rrow_start[k]; e < nl.rrow_start[k + 1]; ++e) {
+            const int c = nl.cidx[nl.rnbr[e]];
+            if (c < 0) continue;
+            const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 2];
+            fx -= a.x;
+            fy -= a.y;
+            fz -= a.z;
+            tx += b.x;
+            ty += b.y;
+            tz += b.z;
+        }
+    }
+    st.fx[k] = fx;
+    st.fy[k] = fy;
+    st.fz[k] = fz;
+    st.tx[k] = tx;
+    st.ty[k] = ty;
+    st.tz[k] = tz;
+    st.upot[k] = u;
+}
+
+static int check_state(const nepx_state* st, const nepx_nlist* nl) {
+    NEPX_REQUIRE(st && nl, NEPX_EINVAL, "null state or list");
+    NEPX_REQUIRE(st->n > 0 && st->n_sys > 0, NEPX_EINVAL, "empty state");
+    NEPX_REQUIRE(st->x && st->y && st->z && st->sys && st->sp, NEPX_EINVAL, "null state buffer");
+    NEPX_REQUIRE(nl->flags && nl->nbr && nl->slot_row && nl->rnbr && nl->row_start && nl->rrow_start &&
+                     nl->row_count && nl->rrow_count && nl->cidx && nl->pair_i && nl->pair_j && nl->pair_t &&
+                     nl->blk_count,
+                 NEPX_EINVAL, "null list buffer");
+    NEPX_REQUIRE(nl->cap > 0 && nl->cap < (int64_t)1 << 31, NEPX_ECAP, "list capacity %lld out of range",
+                 (long long)nl->cap);
+    return NEPX_OK;
+}
+
+static int blocks_for(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace nepx
+
+using namespace nepx;
+
+extern "C" int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, void* stream) {
+    int rc = check_state(st, nl);
+    if (rc) return rc;
+    NEPX_REQUIRE(nl->cell_count && nl->cell_start && nl->cell_fill && nl->cell_list && nl->cell_of, NEPX_EINVAL,
+                 "null cell buffer");
+    hipStream_t s = as_stream(stream);
+    const int n = st->n;
+    const int gb = blocks_for(n, kNB);
+    if (force) {  // a forced build starts clean: request it and clear a stale overflow
+        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_REBUILD, 1);
+        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_OVERFLOW, 0);
+        NEPX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_MAXROW, 0);
+    const int nct = nl->n_cells_total > 0 ? nl->n_cells_total : 1;
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(nct, kNB)), dim3(kNB), 0, s, nl->cell_count, (int64_t)nct,
+                       nl->flags);
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(nct, kNB)), dim3(kNB), 0, s, nl->cell_fill, (int64_t)nct,
+                       nl->flags);
+    hipLaunchKernelGGL(k_cells_count, dim3(gb), dim3(kNB), 0, s, *st, *nl);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(kScanBlock), 0, s, *nl);
+    hipLaunchKernelGGL(k_cells_fill, dim3(gb), dim3(kNB), 0, s, *st, *nl);
+    hipLaunchKernelGGL(k_nbr_count, dim3(gb), dim3(kNB), 0, s, *st, *nl);
+    hipLaunchKernelGGL(k_scan_rows, dim3(2), dim3(kScanBlock), 0, s, *nl, n);
+    hipLaunchKernelGGL(k_nbr_fill, dim3(gb), dim3(kNB), 0, s, *st, *nl);
+    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 0);
+    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 1);
+    hipLaunchKernelGGL(k_rev_slots, dim3(gb), dim3(kNB), 0, s, *nl, n);
+    hipLaunchKernelGGL(k_build_done, dim3(gb), dim3(kNB), 0, s, *st, *nl);
+    hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
+extern "C" int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64_t* pairs, int64_t m,
+                                     void* stream) {
+    int rc = check_state(st, nl);
+    if (rc) return rc;
+    NEPX_REQUIRE(m >= 0 && (m == 0 || pairs), NEPX_EINVAL, "bad pair array");
+    NEPX_REQUIRE(m <= nl->cap, NEPX_ECAP, "%lld pairs exceed list capacity %lld", (long long)m,
+                 (long long)nl->cap);
+    NEPX_REQUIRE(nl->cell_fill && nl->cell_count, NEPX_EINVAL, "null scratch buffer");
+    hipStream_t s = as_stream(stream);
+    const int n = st->n;
+    // flags: request a "build" so the shared kernels run, clear overflow
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_REBUILD, 1);
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_OVERFLOW, 0);
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_MAXROW, 0);
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(n + 1, kNB)), dim3(kNB), 0, s, nl->row_count,
+                       (int64_t)(n + 1), nullptr);
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(n + 1, kNB)), dim3(kNB), 0, s, nl->rrow_count,
+                       (int64_t)(n + 1), nullptr);
+    if (m > 0)
+        hipLaunchKernelGGL(k_pairs_count, dim3(blocks_for(m, kNB)), dim3(kNB), 0, s, *nl, pairs, m, n);
+    hipLaunchKernelGGL(k_rows_maxlen, dim3(blocks_for(n, kNB)), dim3(kNB), 0, s, *nl, n);
+    hipLaunchKernelGGL(k_scan_rows, dim3(2), dim3(kScanBlock), 0, s, *nl, n);
+    // per-row fill counters reuse the cell scratch when it is large enough
+    NEPX_REQUIRE(nl->n_cells_total >= 2 * n, NEPX_ECAP, "cell scratch (%d) smaller than 2*n (%d)",
+                 nl->n_cells_total, 2 * n);
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(2 * n, kNB)), dim3(kNB), 0, s, nl->cell_fill,
+                       (int64_t)(2 * n), nullptr);
+    if (m > 0)
+        hipLaunchKernelGGL(k_pairs_fill, dim3(blocks_for(m, kNB)), dim3(kNB), 0, s, *nl, pairs, m,
+                           nl->cell_fill, nl->cell_fill + n);
+    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 0);
+    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 1);
+    hipLaunchKernelGGL(k_rev_slots, dim3(blocks_for(n, kNB)), dim3(kNB), 0, s, *nl, n);
+    hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
+extern "C" int nepx_pairs_filter(nepx_state* st, nepx_nlist* nl, void* stream) {
+    int rc = check_state(st, nl);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    const int nblk = blocks_for(nl->cap, kSlotsPerBlock);
+    hipLaunchKernelGGL(k_filter_count, dim3(nblk), dim3(kNB), 0, s, *st, *nl);
+    hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(kScanBlock), 0, s, *nl, nblk);
+    hipLaunchKernelGGL(k_filter_write, dim3(nblk), dim3(kNB), 0, s, *st, *nl);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
+extern "C" int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* pair_out, void* stream) {
+    int rc = check_state(st, nl);
+    if (rc) return rc;
+    NEPX_REQUIRE(pair_out && st->fx && st->fy && st->fz && st->tx && st->ty && st->tz && st->upot, NEPX_EINVAL,
+                 "null force buffer");
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(st->n, kNB)), dim3(kNB), 0, as_stream(stream), *st, *nl,
+                       (const float4*)pair_out);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}

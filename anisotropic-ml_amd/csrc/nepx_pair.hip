@@ -1,0 +1,432 @@
+// Pair kernels: descriptor forward, MFMA energy network, adjoint, and the
+// fused step kernel.  See nepx_desc.h and nepx_ffnn.h for the math and layouts.
+#include "nepx_desc.h"
+#include "nepx_ffnn.h"
+
+#include <string.h>
+
+namespace nepx {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ int n_pairs(const int32_t* np_dev, int64_t maxp) {
+    const int p = *np_dev;
+    return p < maxp ? p : (int)maxp;
+}
+
+__device__ __forceinline__ void load_pair(const float4* __restrict__ qf, const int32_t* __restrict__ pi,
+                                          const int32_t* __restrict__ pj, const float4* __restrict__ pt,
+                                          int idx, float4& qi, float4& qj, float4& t) {
+    const int i = pi[idx];
+    const int j = pj[idx];
+    t = pt[idx];
+    qi = qf[i];
+    qj = qf[j];
+}
+
+template <class Cfg>
+__device__ __forceinline__ void store_row(float* __restrict__ dst, const float (&v)[Cfg::D]) {
+    if constexpr (Cfg::D % 4 == 0) {
+        float4* d4 = reinterpret_cast<float4*>(dst);
+#pragma unroll
+        for (int k = 0; k < Cfg::D / 4; ++k) d4[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < Cfg::D; ++k) dst[k] = v[k];
+    }
+}
+
+template <class Cfg>
+__device__ __forceinline__ void load_row(const float* __restrict__ src, float (&v)[Cfg::D]) {
+    if constexpr (Cfg::D % 4 == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+#pragma unroll
+        for (int k = 0; k < Cfg::D / 4; ++k) {
+            const float4 x = s4[k];
+            v[4 * k] = x.x;
+            v[4 * k + 1] = x.y;
+            v[4 * k + 2] = x.z;
+            v[4 * k + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < Cfg::D; ++k) v[k] = src[k];
+    }
+}
+
+__device__ __forceinline__ void store_out(float* __restrict__ out, int idx, const PairOut& o, float E,
+                                          float clip) {
+    float4* o4 = reinterpret_cast<float4*>(out) + 3 * (size_t)idx;
+    o4[0] = make_float4(clipf(o.fx, clip), clipf(o.fy, clip), clipf(o.fz, clip), E);
+    o4[1] = make_float4(clipf(o.tix, clip), clipf(o.tiy, clip), clipf(o.tiz, clip), 0.f);
+    o4[2] = make_float4(clipf(o.tjx, clip), clipf(o.tjy, clip), clipf(o.tjz, clip), 0.f);
+}
+
+// ---------------------------------------------------------------------------
+template <class Cfg>
+__global__ __launch_bounds__(kBlock) void k_desc_fwd(nepx_pair_model m, const float4* __restrict__ qf,
+                                                     const int32_t* __restrict__ pi,
+                                                     const int32_t* __restrict__ pj,
+                                                     const float4* __restrict__ pt,
+                                                     const int32_t* __restrict__ np_dev, int64_t maxp,
+                                                     float* __restrict__ G) {
+    const int P = n_pairs(np_dev, maxp);
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < P; idx += gridDim.x * kBlock) {
+        float4 qi, qj, t;
+        load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+        const PairGeom g = pair_geom<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        float grad[Cfg::NRAD + 1];
+        float A[Cfg::NANG + 1][Cfg::NLM];
+        desc_accumulate<Cfg>(m, g, grad, A);
+        float Gr[Cfg::D];
+        desc_finish<Cfg>(grad, A, Gr);
+        store_row<Cfg>(G + (size_t)idx * Cfg::D, Gr);
+    }
+}
+
+template <class Cfg>
+__global__ __launch_bounds__(kBlock) void k_adjoint(nepx_pair_model m, const float4* __restrict__ qf,
+                                                    const int32_t* __restrict__ pi,
+                                                    const int32_t* __restrict__ pj,
+                                                    const float4* __restrict__ pt,
+                                                    const int32_t* __restrict__ np_dev, int64_t maxp,
+                                                    const float* __restrict__ dEdG,
+                                                    const float* __restrict__ E, float clip,
+                                                    float* __restrict__ out) {
+    const int P = n_pairs(np_dev, maxp);
+    for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < P; idx += gridDim.x * kBlock) {
+        float4 qi, qj, t;
+        load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+        const PairGeom g = pair_geom<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        float grad[Cfg::NRAD + 1];
+        float A[Cfg::NANG + 1][Cfg::NLM];
+        desc_accumulate<Cfg>(m, g, grad, A);
+        float dG[Cfg::D];
+        load_row<Cfg>(dEdG + (size_t)idx * Cfg::D, dG);
+        const PairOut o = desc_adjoint<Cfg>(m, g, dG, A);
+        store_out(out, idx, o, E ? E[idx] : 0.f, clip);
+    }
+}
+
+// Input chunks of one pair for the MFMA tile: chunk c holds inputs
+// 16c + 4*(lane/16) + 0..3 (zero past D).
+template <class N>
+__device__ __forceinline__ void gather_chunks(const float* __restrict__ row, bool valid, int D,
+                                              v4f (&gin)[N::MD]) {
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int c = 0; c < N::MD; ++c) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = 16 * c + 4 * g + r;
+            gin[c][r] = (valid && k < D) ? row[k] : 0.f;
+        }
+    }
+}
+
+template <class N>
+__device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool valid, int D,
+                                               const v4f (&v)[N::MD]) {
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int c = 0; c < N::MD; ++c) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = 16 * c + 4 * g + r;
+            if (valid && k < D) row[k] = v[c][r];
+        }
+    }
+}
+
+template <class N>
+__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ W, int act, float en_min,
+                                                 float scale, int D, const float* __restrict__ G,
+                                                 const int32_t* __restrict__ np_dev, int64_t maxp,
+                                                 float* __restrict__ E, float* __restrict__ dEdG) {
+    const int P = n_pairs(np_dev, maxp);
+    const int lane = threadIdx.x & 63;
+    const int nt = (P + 15) / 16;
+    const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * kBlock) >> 6;
+    for (int tile = wave; tile < nt; tile += nwaves) {
+        const int p = tile * 16 + (lane & 15);
+        const bool valid = p < P;
+        v4f gin[N::MD];
+        gather_chunks<N>(G + (size_t)p * D, valid, D, gin);
+        v4f dg[N::MD];
+        const float y = ffnn_tile<N>(W, act, gin, scale, dg);
+        if (valid && lane < 16) E[p] = en_min + scale * y;
+        scatter_chunks<N>(dEdG + (size_t)p * D, valid, D, dg);
+    }
+}
+
+// Fused: descriptor -> LDS -> MFMA network -> LDS -> adjoint.  Each block
+// takes 256 consecutive pairs per iteration (64 per wave), so every wave of a
+// block runs the same number of iterations and the barriers are uniform.
+template <class Cfg, class N>
+__global__ __launch_bounds__(kBlock) void k_fused(nepx_pair_model m, const float* __restrict__ W,
+                                                  int act, float en_min, float scale,
+                                                  const float4* __restrict__ qf,
+                                                  const int32_t* __restrict__ pi,
+                                                  const int32_t* __restrict__ pj,
+                                                  const float4* __restrict__ pt,
+                                                  const int32_t* __restrict__ np_dev, int64_t maxp,
+                                                  float clip, float* __restrict__ out) {
+    constexpr int LD = (Cfg::D + 3) / 4 * 4 + 4;  // row stride in LDS (16-byte aligned)
+    __shared__ __attribute__((aligned(16))) float tile[kBlock * LD];
+    __shared__ float etile[kBlock];
+    const int P = n_pairs(np_dev, maxp);
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    float* myrow = tile + threadIdx.x * LD;
+    for (int base = blockIdx.x * kBlock; base < P; base += gridDim.x * kBlock) {
+        const int idx = base + threadIdx.x;
+        const bool valid = idx < P;
+        float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
+        if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+        const PairGeom g = pair_geom<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        {
+            float grad[Cfg::NRAD + 1];
+            float A[Cfg::NANG + 1][Cfg::NLM];
+            desc_accumulate<Cfg>(m, g, grad, A);
+            float Gr[Cfg::D];
+            desc_finish<Cfg>(grad, A, Gr);
+#pragma unroll
+            for (int k = 0; k < Cfg::D; ++k) myrow[k] = valid ? Gr[k] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int sub = 0; sub < 4; ++sub) {
+            const int local = wv * 64 + sub * 16 + (lane & 15);
+            float* row = tile + local * LD;
+            v4f gin[N::MD];
+            gather_chunks<N>(row, true, Cfg::D, gin);
+            v4f dg[N::MD];
+            const float y = ffnn_tile<N>(W, act, gin, scale, dg);
+            if (lane < 16) etile[local] = en_min + scale * y;
+            scatter_chunks<N>(row, true, Cfg::D, dg);
+        }
+        __syncthreads();
+        if (valid) {
+            float dG[Cfg::D];
+#pragma unroll
+            for (int k = 0; k < Cfg::D; ++k) dG[k] = myrow[k];
+            float grad[Cfg::NRAD + 1];
+            float A[Cfg::NANG + 1][Cfg::NLM];
+            desc_accumulate<Cfg>(m, g, grad, A);
+            const PairOut o = desc_adjoint<Cfg>(m, g, dG, A);
+            store_out(out, idx, o, etile[threadIdx.x], clip);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch tables.
+
+using CfgCube8 = DescCfg<8, 11, 6, 4>;     // cube, 8 vertices, D = 40
+using CfgTet10 = DescCfg<10, 11, 6, 4>;    // tetrahedron, 4 vertices + 6 mid-edges, D = 40
+using CfgCyl12 = DescCfg<12, 17, 6, 6>;    // twisted cylinder, 12 helix sites, D = 60
+using CfgCubeV2 = DescCfg<6, 10, 8, 4>;    // reference cube_v2 with hypers 10 8 4, D = 47
+using CfgThV1 = DescCfg<4, 10, 8, 4>;      // reference th_v1 with hypers 10 8 4, D = 47
+
+using Net40x64 = NetCfg<48, 64, 2>;        // 40 -> 64 -> 64 -> 1
+using Net60x128 = NetCfg<64, 128, 2>;      // 60 -> 128 -> 128 -> 1
+
+static int grid_for(int64_t maxp, int per_block) {
+    int64_t g = (maxp + per_block - 1) / per_block;
+    const int64_t cap = 256 * 16;  // persistent cap: 16 blocks per CU on 256 CUs
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+#define NEPX_DESC_CASES(X)                          \
+    X(8, 11, 6, 4, CfgCube8)                        \
+    X(10, 11, 6, 4, CfgTet10)                       \
+    X(12, 17, 6, 6, CfgCyl12)                       \
+    X(6, 10, 8, 4, CfgCubeV2)                       \
+    X(4, 10, 8, 4, CfgThV1)
+
+static bool desc_supported(const nepx_pair_model* m) {
+#define X(ns, a, b, c, T) \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c) return true;
+    NEPX_DESC_CASES(X)
+#undef X
+    return false;
+}
+
+static int check_model(const nepx_pair_model* m) {
+    NEPX_REQUIRE(m != nullptr, NEPX_EINVAL, "null pair model");
+    NEPX_REQUIRE(m->n_sites > 0 && m->n_sites <= NEPX_MAX_SITES, NEPX_EINVAL, "n_sites %d out of range",
+                 m->n_sites);
+    NEPX_REQUIRE(m->nang <= m->nrad, NEPX_EINVAL, "nang (%d) must not exceed nrad (%d)", m->nang, m->nrad);
+    NEPX_REQUIRE(m->r_nep > 0.f, NEPX_EINVAL, "r_nep must be positive");
+    NEPX_REQUIRE(desc_supported(m), NEPX_EUNSUP,
+                 "no descriptor kernel for n_sites=%d hypers=(%d,%d,%d)", m->n_sites, m->nrad, m->nang,
+                 m->lmax);
+    return NEPX_OK;
+}
+
+static int check_net(const nepx_net* n, int D) {
+    NEPX_REQUIRE(n != nullptr && n->w_packed != nullptr, NEPX_EINVAL, "null network");
+    NEPX_REQUIRE(n->d_in == D, NEPX_EINVAL, "network input %d != descriptor size %d", n->d_in, D);
+    NEPX_REQUIRE(n->act == NEPX_ACT_TANH || n->act == NEPX_ACT_RELU, NEPX_EINVAL, "bad activation %d",
+                 n->act);
+    return NEPX_OK;
+}
+
+static int model_D(const nepx_pair_model* m) { return (m->nrad + 1) + (m->nang + 1) * m->lmax; }
+
+}  // namespace nepx
+
+using namespace nepx;
+
+extern "C" int64_t nepx_net_packed_floats(int32_t d_in, int32_t width, int32_t n_hidden) {
+    if (d_in <= 0 || width <= 0 || n_hidden < 0) return -1;
+    NetLayout L{round16(d_in), round16(width), n_hidden + 1};
+    return L.total();
+}
+
+extern "C" int nepx_net_pack(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
+                             const float* const* bs, float* out) {
+    NEPX_REQUIRE(d_in > 0 && width > 0 && n_hidden >= 0, NEPX_EINVAL, "bad network dims");
+    NEPX_REQUIRE(ws && bs && out, NEPX_EINVAL, "null pointer");
+    const int DP = round16(d_in), HP = round16(width), NL = n_hidden + 1;
+    NetLayout L{DP, HP, NL};
+    memset(out, 0, sizeof(float) * L.total());
+    for (int k = 0; k < NL; ++k) {
+        const int kin = k == 0 ? d_in : width, kinp = k == 0 ? DP : HP;
+        const float* W = ws[k];  // (width x kin)
+        NEPX_REQUIRE(W && bs[k], NEPX_EINVAL, "null layer %d", k);
+        // forward operand: [mt][s][lane] = W[16mt + lane%16][16(s/4) + 4(lane/16) + s%4]
+        float* F = out + L.fwd_off(k);
+        for (int mt = 0; mt < HP / 16; ++mt)
+            for (int s = 0; s < kinp / 4; ++s)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int row = 16 * mt + lane % 16;
+                    const int col = 16 * (s / 4) + 4 * (lane / 16) + s % 4;
+                    F[(mt * (kinp / 4) + s) * 64 + lane] = (row < width && col < kin) ? W[row * kin + col] : 0.f;
+                }
+        // backward operand (W^T): [mt][s][lane] = W[16(s/4) + 4(lane/16) + s%4][16mt + lane%16]
+        float* B = out + L.bwd_off(k);
+        for (int mt = 0; mt < kinp / 16; ++mt)
+            for (int s = 0; s < HP / 4; ++s)
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int row = 16 * (s / 4) + 4 * (lane / 16) + s % 4;
+                    const int col = 16 * mt + lane % 16;
+                    B[(mt * (HP / 4) + s) * 64 + lane] = (row < width && col < kin) ? W[row * kin + col] : 0.f;
+                }
+        for (int u = 0; u < width; ++u) out[L.bias_off(k) + u] = bs[k][u];
+    }
+    NEPX_REQUIRE(ws[NL] && bs[NL], NEPX_EINVAL, "null output layer");
+    for (int u = 0; u < width; ++u) out[L.wout_off() + u] = ws[NL][u];
+    out[L.bout_off()] = bs[NL][0];
+    return NEPX_OK;
+}
+
+extern "C" int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
+                                   const int32_t* pair_j, const float* pair_t, const int32_t* np_dev,
+                                   int64_t maxp, float* G, void* stream) {
+    int rc = check_model(m);
+    if (rc) return rc;
+    NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && G, NEPX_EINVAL, "null buffer");
+    if (maxp <= 0) return NEPX_OK;
+    const int grid = grid_for(maxp, kBlock);
+#define X(ns, a, b, c, T)                                                                           \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c) {                        \
+        hipLaunchKernelGGL(k_desc_fwd<T>, dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,       \
+                           (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, G); \
+        NEPX_LAUNCH_CHECK();                                                                        \
+        return NEPX_OK;                                                                             \
+    }
+    NEPX_DESC_CASES(X)
+#undef X
+    return NEPX_EUNSUP;
+}
+
+#define NEPX_NET_CASES(X)      \
+    X(40, 64, 1, Net40x64)     \
+    X(60, 128, 1, Net60x128)
+
+static bool net_supported(const nepx_net* n) {
+#define X(d, w, h, T) if (n->d_in == d && n->width == w && n->n_hidden == h) return true;
+    NEPX_NET_CASES(X)
+#undef X
+    return false;
+}
+
+extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int32_t* np_dev, int64_t maxp,
+                                 float* E, float* dEdG, void* stream) {
+    NEPX_REQUIRE(net && net->w_packed, NEPX_EINVAL, "null network");
+    NEPX_REQUIRE(G && np_dev && E && dEdG, NEPX_EINVAL, "null buffer");
+    NEPX_REQUIRE(net_supported(net), NEPX_EUNSUP, "no network kernel for %d -> %d x%d", net->d_in,
+                 net->width, net->n_hidden + 1);
+    if (maxp <= 0) return NEPX_OK;
+    const float scale = net->en_max - net->en_min;
+    const int grid = grid_for((maxp + 15) / 16, kBlock / 64);
+#define X(d, w, h, T)                                                                               \
+    if (net->d_in == d && net->width == w && net->n_hidden == h) {                                  \
+        hipLaunchKernelGGL(k_ffnn<T>, dim3(grid), dim3(kBlock), 0, as_stream(stream), net->w_packed, \
+                           net->act, net->en_min, scale, d, G, np_dev, maxp, E, dEdG);              \
+        NEPX_LAUNCH_CHECK();                                                                        \
+        return NEPX_OK;                                                                             \
+    }
+    NEPX_NET_CASES(X)
+#undef X
+    return NEPX_EUNSUP;
+}
+
+extern "C" int nepx_descriptor_adjoint(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
+                                       const int32_t* pair_j, const float* pair_t, const int32_t* np_dev,
+                                       int64_t maxp, const float* dEdG, const float* E, float clip,
+                                       float* out, void* stream) {
+    int rc = check_model(m);
+    if (rc) return rc;
+    NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && dEdG && out, NEPX_EINVAL, "null buffer");
+    if (maxp <= 0) return NEPX_OK;
+    const int grid = grid_for(maxp, kBlock);
+#define X(ns, a, b, c, T)                                                                            \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c) {                         \
+        hipLaunchKernelGGL(k_adjoint<T>, dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,         \
+                           (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, dEdG, \
+                           E, clip, out);                                                            \
+        NEPX_LAUNCH_CHECK();                                                                         \
+        return NEPX_OK;                                                                              \
+    }
+    NEPX_DESC_CASES(X)
+#undef X
+    return NEPX_EUNSUP;
+}
+
+#define NEPX_FUSED_CASES(X)                      \
+    X(8, 11, 6, 4, CfgCube8, 40, 64, 1, Net40x64) \
+    X(10, 11, 6, 4, CfgTet10, 40, 64, 1, Net40x64) \
+    X(12, 17, 6, 6, CfgCyl12, 60, 128, 1, Net60x128)
+
+extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
+                               const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                               const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+    int rc = check_model(m);
+    if (rc) return rc;
+    rc = check_net(net, model_D(m));
+    if (rc) return rc;
+    NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && out, NEPX_EINVAL, "null buffer");
+    if (maxp <= 0) return NEPX_OK;
+    const float scale = net->en_max - net->en_min;
+    const int grid = grid_for(maxp, kBlock);
+#define X(ns, a, b, c, T, d, w, h, NT)                                                                \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&         \
+        net->width == w && net->n_hidden == h) {                                                      \
+        hipLaunchKernelGGL((k_fused<T, NT>), dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,       \
+                           net->w_packed, net->act, net->en_min, scale, (const float4*)qf, pair_i,     \
+                           pair_j, (const float4*)pair_t, np_dev, maxp, clip, out);                   \
+        NEPX_LAUNCH_CHECK();                                                                          \
+        return NEPX_OK;                                                                               \
+    }
+    NEPX_FUSED_CASES(X)
+#undef X
+    set_error("no fused kernel for n_sites=%d hypers=(%d,%d,%d) net %d->%dx%d", m->n_sites, m->nrad,
+              m->nang, m->lmax, net->d_in, net->width, net->n_hidden + 1);
+    return NEPX_EUNSUP;
+}

@@ -1,0 +1,98 @@
+"""Step runner: a dedicated HIP stream, optional hipGraph capture, kernel timing events.
+
+The step is split into three device-only segments so that the dominant
+kernel can be timed with events on the stream it runs on:
+  pre   = conditional list rebuild + COM filter/compaction
+  pair  = fused descriptor -> MFMA network -> adjoint kernel
+  post  = deterministic gather + FP64 NVT update + thermostat/rebuild test
+With use_graph each segment is captured once into a hipGraph (through
+torch.cuda.CUDAGraph, which drives hipStreamBeginCapture) and replayed; no
+segment contains hipMemsetAsync (all clears are fill kernels).
+"""
+import torch
+
+from .system import Arena
+
+
+class StepRunner:
+    def __init__(self, arena: Arena, use_graph=True, stream=None):
+        self.ar = arena
+        self.use_graph = use_graph
+        self.stream = stream or torch.cuda.Stream(device=arena.device)
+        self.graphs = None
+        self._ev = None
+        self._ev_i = 0
+
+    def init(self):
+        with torch.cuda.stream(self.stream):
+            self.ar.init(self.stream)
+        self.stream.synchronize()
+        if self.use_graph:
+            self._capture()
+
+    # segments ---------------------------------------------------------------
+    def _pre(self):
+        self.ar.build_list(False, self.stream)
+        self.ar.filter_pairs(self.stream)
+
+    def _pair(self):
+        self.ar.pair_fused(self.stream)
+
+    def _post(self):
+        self.ar.gather(self.stream)
+        self.ar.nvt_step(self.stream)
+
+    def _capture(self):
+        # warm the segments once eagerly (one real step), then capture
+        self._eager_step()
+        self.stream.synchronize()
+        self.graphs = []
+        for seg in (self._pre, self._pair, self._post):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self.stream):
+                seg()
+            self.graphs.append(g)
+        self.stream.synchronize()
+
+    def _eager_step(self):
+        with torch.cuda.stream(self.stream):
+            self._pre()
+            self._timed(self._pair)
+            self._post()
+        self.ar.steps += 1
+
+    def _timed(self, fn):
+        if self._ev is not None and self._ev_i < len(self._ev):
+            a, b = self._ev[self._ev_i]
+            a.record(self.stream)
+            fn()
+            b.record(self.stream)
+            self._ev_i += 1
+        else:
+            fn()
+
+    def step(self):
+        if not self.graphs:
+            self._eager_step()
+            return
+        with torch.cuda.stream(self.stream):
+            self.graphs[0].replay()
+            self._timed(self.graphs[1].replay)
+            self.graphs[2].replay()
+        self.ar.steps += 1
+
+    def run(self, n):
+        for _ in range(n):
+            self.step()
+
+    # timing -----------------------------------------------------------------
+    def enable_kernel_events(self, n):
+        self._ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+        self._ev_i = 0
+        return self._ev
+
+    def kernel_event_ms(self):
+        """Average duration of the timed segment over the recorded steps (synchronises)."""
+        self.stream.synchronize()
+        ts = [a.elapsed_time(b) for a, b in self._ev[: self._ev_i]]
+        return sum(ts) / max(len(ts), 1)

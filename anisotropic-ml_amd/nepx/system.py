@@ -1,0 +1,265 @@
+"""Device-resident state of one or more rigid-body systems and the MD step.
+
+Layout in HBM (all structure-of-arrays, FP64 unless noted):
+  x y z | vx vy vz | q0..q3 | p0..p3 | mass | fx fy fz tx ty tz upot | xb yb zb
+  qf (float4 copy of q for the pair kernels), sys (int32 system id, -1 = padding)
+Each system occupies a block-aligned particle range (256 particles per
+integrator block), so per-system reductions never straddle a block.
+The neighbour list, compacted pair arrays and per-pair outputs live beside it.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import EnergyNet, make_pair_model, n_descriptor
+from .shapes import Shape
+
+BLOCK = 256
+REF_THERMO_INIT = dict(xi=-0.0317711, eta=0.108609, xi_r=0.00814227, eta_r=-0.014238)  # Sim.py:99-111
+
+
+def _align(n, b=BLOCK):
+    return (n + b - 1) // b * b
+
+
+class SystemSpec:
+    """Host description of one system (FP64 numpy arrays)."""
+
+    def __init__(self, x, v, q, p, mass, L, kT, dt, moi, rc, skin=0.5, tau=None, thermo=None,
+                 ndof=None, rndof=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = len(self.x)
+        self.v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3)
+        self.q = np.ascontiguousarray(q, dtype=np.float64).reshape(n, 4)
+        self.p = np.ascontiguousarray(p, dtype=np.float64).reshape(n, 4)
+        self.mass = np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,)).copy()
+        self.L = float(L)
+        self.kT = float(kT)
+        self.dt = float(dt)
+        self.tau = 100.0 * self.dt if tau is None else float(tau)   # Sim.setdt (Sim.py:113-115)
+        self.moi = tuple(float(m) for m in moi)
+        self.rc = float(rc)
+        self.skin = float(skin)
+        self.thermo = dict(REF_THERMO_INIT if thermo is None else thermo)
+        self.ndof = 3.0 * (n - 1) if ndof is None else float(ndof)   # Sim.py:35-36
+        self.rndof = 3.0 * n if rndof is None else float(rndof)
+
+    @property
+    def n(self):
+        return len(self.x)
+
+
+class Arena:
+    """One or more systems sharing a pair model and energy network, resident on one GPU."""
+
+    def __init__(self, specs, shape: Shape, hypers, net: EnergyNet, device="cuda", clip=100.0,
+                 list_cap=None, r_nep=None):
+        if not torch.cuda.is_available():
+            raise _lib.NepxError("nepx needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device(device)
+        self.specs = list(specs)
+        self.shape = shape
+        self.hypers = tuple(int(h) for h in hypers[:3])
+        self.D = n_descriptor(*self.hypers)
+        if net.d_in != self.D:
+            raise ValueError(f"network input {net.d_in} != descriptor size {self.D}")
+        self.net = net
+        self.model = make_pair_model(shape, self.hypers, r_nep)
+        self.cnet = net.cstruct()
+        self.clip = float(clip)
+        self._layout()
+        self._alloc_state()
+        self._alloc_list(list_cap)
+        self.steps = 0
+
+    # ---------------------------------------------------------------- layout
+    def _layout(self):
+        starts, off = [], 0
+        for sp in self.specs:
+            starts.append(off)
+            off += _align(sp.n)
+        self.n = max(off, BLOCK)
+        self.starts = starts
+        rec = np.zeros(len(self.specs), dtype=_lib.SYSPARAM_DTYPE)
+        cell_off = 0
+        for k, sp in enumerate(self.specs):
+            r_list = sp.rc + sp.skin
+            if sp.L < 2.0 * sp.rc:
+                raise ValueError(f"box {sp.L} smaller than twice the cutoff {sp.rc}")
+            nc = int(math.floor(sp.L / r_list))
+            nc = nc if nc >= 3 else 0
+            r = rec[k]
+            r["L"], r["kT"], r["dt"], r["tau"] = sp.L, sp.kT, sp.dt, sp.tau
+            r["ndof"], r["rndof"] = sp.ndof, sp.rndof
+            r["moi"] = sp.moi
+            r["r_list"], r["rc"], r["skin"] = r_list, sp.rc, sp.skin
+            r["p_start"], r["p_end"] = starts[k], starts[k] + sp.n
+            r["blk_start"], r["blk_end"] = starts[k] // BLOCK, (starts[k] + sp.n + BLOCK - 1) // BLOCK
+            r["nc"], r["cell_off"] = nc, cell_off
+            cell_off += nc ** 3
+        self.sysrec = rec
+        self.n_cells = max(cell_off, 2 * self.n)
+
+    def _alloc_state(self):
+        dev, n = self.device, self.n
+        f64 = lambda: torch.zeros(n, dtype=torch.float64, device=dev)
+        self.t = {k: f64() for k in ["x", "y", "z", "vx", "vy", "vz", "q0", "q1", "q2", "q3", "p0", "p1", "p2",
+                                     "p3", "mass", "fx", "fy", "fz", "tx", "ty", "tz", "upot", "xb", "yb",
+                                     "zb"]}
+        self.t["qf"] = torch.zeros(n * 4, dtype=torch.float32, device=dev)
+        host = {k: np.zeros(n) for k in ["x", "y", "z", "vx", "vy", "vz", "q0", "q1", "q2", "q3", "p0", "p1",
+                                         "p2", "p3"]}
+        host["q0"][:] = 1.0
+        mass = np.ones(n)
+        sysid = np.full(n, -1, dtype=np.int32)
+        for k, sp in enumerate(self.specs):
+            a, b = self.starts[k], self.starts[k] + sp.n
+            for c, key in enumerate("xyz"):
+                host[key][a:b] = sp.x[:, c]
+                host["v" + key][a:b] = sp.v[:, c]
+            for c in range(4):
+                host[f"q{c}"][a:b] = sp.q[:, c]
+                host[f"p{c}"][a:b] = sp.p[:, c]
+            mass[a:b] = sp.mass
+            sysid[a:b] = k
+        for k, v in host.items():
+            self.t[k].copy_(torch.from_numpy(v))
+        self.t["mass"].copy_(torch.from_numpy(mass))
+        self.t["sys"] = torch.from_numpy(sysid).to(dev)
+        self.t["sp"] = torch.from_numpy(self.sysrec.view(np.uint8).copy()).to(dev)
+        th = np.zeros(len(self.specs), dtype=_lib.THERMO_DTYPE)
+        for k, sp in enumerate(self.specs):
+            for key, val in sp.thermo.items():
+                th[k][key] = val
+            th[k]["theta"] = 1.0
+            th[k]["e_r"] = 1.0
+        self.t["th"] = torch.from_numpy(th.view(np.uint8).copy()).to(dev)
+        self.t["partials"] = torch.zeros(4 * (n // BLOCK + 1), dtype=torch.float64, device=dev)
+        st = _lib.State()
+        st.n, st.n_sys = n, len(self.specs)
+        for k in _lib._STATE_PTRS:
+            setattr(st, k, self.t[k].data_ptr())
+        self.st = st
+
+    def _estimate_cap(self):
+        tot = 0.0
+        for sp in self.specs:
+            rho = sp.n / sp.L ** 3
+            tot += 0.5 * sp.n * min(sp.n - 1, rho * 4.0 / 3.0 * math.pi * (sp.rc + sp.skin) ** 3)
+        return int(1.3 * tot) + 4096
+
+    def _alloc_list(self, cap=None):
+        dev, n = self.device, self.n
+        cap = int(cap or self._estimate_cap())
+        i32 = lambda m: torch.zeros(m, dtype=torch.int32, device=dev)
+        L = {
+            "cell_count": i32(self.n_cells + 1), "cell_start": i32(self.n_cells + 1),
+            "cell_fill": i32(self.n_cells + 1), "cell_list": i32(n), "cell_of": i32(n),
+            "row_count": i32(n + 1), "row_start": i32(n + 1), "rrow_count": i32(n + 1),
+            "rrow_start": i32(n + 1), "nbr": i32(cap), "slot_row": i32(cap), "rnbr": i32(cap),
+            "cidx": i32(cap), "blk_count": i32(cap // 1024 + 4), "flags": i32(_lib.N_FLAGS),
+            "pair_i": i32(cap), "pair_j": i32(cap),
+            "pair_t": torch.zeros(cap * 4, dtype=torch.float32, device=dev),
+        }
+        self.lt = L
+        self.pair_out = torch.zeros(cap * 12, dtype=torch.float32, device=dev)
+        nl = _lib.NList()
+        nl.cap = cap
+        nl.n_cells_total = self.n_cells
+        for k in _lib._NL_PTRS:
+            setattr(nl, k, L[k].data_ptr())
+        self.nl = nl
+        self.cap = cap
+
+    # ---------------------------------------------------------------- steps
+    def _s(self, stream=None):
+        return _lib.stream_handle(stream)
+
+    def build_list(self, force=True, stream=None):
+        _lib.check(_lib.lib.nepx_nlist_build(C.byref(self.st), C.byref(self.nl), int(force), self._s(stream)),
+                   "nepx_nlist_build")
+
+    def flags(self):
+        return self.lt["flags"].cpu().numpy()
+
+    def ensure_capacity(self):
+        """Synchronising check after a build: grow the list if it overflowed or is nearly full."""
+        f = self.flags()
+        pl = int(f[_lib.FLAG_PL])
+        if f[_lib.FLAG_MAXROW] > 2048:
+            raise _lib.NepxError(f"a particle has {int(f[_lib.FLAG_MAXROW])} neighbours in the list "
+                                 "radius (maximum 2048)")
+        if f[_lib.FLAG_OVERFLOW] or pl > 0.9 * self.cap:
+            self._alloc_list(int(1.5 * max(pl, self.cap)) + 4096)
+            self.build_list(True)
+            f = self.flags()
+            if f[_lib.FLAG_OVERFLOW]:
+                raise _lib.NepxError(f"neighbour list overflow persists (flags {f.tolist()})")
+        return int(self.flags()[_lib.FLAG_PL])
+
+    def init(self, stream=None):
+        """First list build (with capacity check) and initial kinetic energies + thermostat update."""
+        self.build_list(True, stream)
+        self.ensure_capacity()
+        _lib.check(_lib.lib.nepx_nvt_init(C.byref(self.st), C.byref(self.nl), self._s(stream)), "nepx_nvt_init")
+
+    def step(self, stream=None):
+        _lib.check(_lib.lib.nepx_md_step(C.byref(self.st), C.byref(self.nl), C.byref(self.model),
+                                         C.byref(self.cnet), self.clip, _lib.ptr(self.pair_out),
+                                         self._s(stream)), "nepx_md_step")
+        self.steps += 1
+
+    def check(self):
+        f = self.flags()
+        if f[_lib.FLAG_OVERFLOW]:
+            raise _lib.NepxError(f"neighbour list overflow (flags {f.tolist()}); results since the last "
+                                 "check are invalid")
+        return f
+
+    # ----------------------------------------------------------- components
+    def filter_pairs(self, stream=None):
+        _lib.check(_lib.lib.nepx_pairs_filter(C.byref(self.st), C.byref(self.nl), self._s(stream)),
+                   "nepx_pairs_filter")
+
+    def n_pairs(self):
+        return int(self.lt["flags"][_lib.FLAG_P].item())
+
+    def pairs(self):
+        P = self.n_pairs()
+        return torch.stack([self.lt["pair_i"][:P], self.lt["pair_j"][:P]], dim=1)
+
+    def gather(self, stream=None):
+        _lib.check(_lib.lib.nepx_forces_gather(C.byref(self.st), C.byref(self.nl), _lib.ptr(self.pair_out),
+                                               self._s(stream)), "nepx_forces_gather")
+
+    def pair_fused(self, stream=None):
+        np_dev = C.c_void_p(self.lt["flags"].data_ptr() + 4 * _lib.FLAG_P)
+        _lib.check(_lib.lib.nepx_pair_fused(C.byref(self.model), C.byref(self.cnet), _lib.ptr(self.t["qf"]),
+                                            _lib.ptr(self.lt["pair_i"]), _lib.ptr(self.lt["pair_j"]),
+                                            _lib.ptr(self.lt["pair_t"]), np_dev, self.cap, self.clip,
+                                            _lib.ptr(self.pair_out), self._s(stream)), "nepx_pair_fused")
+
+    def nvt_step(self, stream=None):
+        _lib.check(_lib.lib.nepx_nvt_step(C.byref(self.st), C.byref(self.nl), self._s(stream)), "nepx_nvt_step")
+
+    # --------------------------------------------------------------- access
+    def thermo(self):
+        return self.t["th"].cpu().numpy().view(_lib.THERMO_DTYPE).copy()
+
+    def system_arrays(self, k=0):
+        """FP64 host copies of system k: x (N,3), v, q (N,4), p, F, T, upot."""
+        a, b = self.starts[k], self.starts[k] + self.specs[k].n
+        g = lambda key: self.t[key][a:b].cpu().numpy()
+        return dict(
+            x=np.stack([g("x"), g("y"), g("z")], 1), v=np.stack([g("vx"), g("vy"), g("vz")], 1),
+            q=np.stack([g("q0"), g("q1"), g("q2"), g("q3")], 1),
+            p=np.stack([g("p0"), g("p1"), g("p2"), g("p3")], 1),
+            F=np.stack([g("fx"), g("fy"), g("fz")], 1), T=np.stack([g("tx"), g("ty"), g("tz")], 1),
+            upot=g("upot"), mass=g("mass"))
+
+    def pair_outputs(self):
+        P = self.n_pairs()
+        return self.pair_out[: 12 * P].view(P, 12)

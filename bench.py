@@ -1,0 +1,154 @@
+"""Benchmark of the nepx MD step on one MI355X.
+
+Workload (default): 32,768 rigid tetrahedra (edge 1, 4 vertex + 6 mid-edge
+sites), packing fraction 0.45, rc = r_nep = 2.5, hypers (11, 6, 4) -> D = 40,
+FFNN 40 -> 64 -> 64 -> 1 tanh with seeded N(0, 1/fan_in) weights, FP32 pair
+math + FP64 integrator, NVT at kT = 1, dt = 0.001.  A "step" is one full MD
+step: list rebuild when the skin is exceeded, COM filter, fused descriptor /
+MFMA network / adjoint kernel, deterministic gather, FP64 NVT update.
+
+Prints one JSON line (see the "value"/"roofline"/"cpu_baseline" keys).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "anisotropic-ml_amd")]
+
+PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def ffnn_flops_per_pair(D, widths):
+    h1, h2 = widths
+    return 4 * (D * h1 + h1 * h2 + h2)
+
+
+def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
+    """Reference algorithm (10-row FD stencil, FP64 numpy, one thread) on a bounded pair sample."""
+    import numpy as np
+    from oracle import nep_oracle as O, reference_step as R
+    sh = O.shape_table()[shape_name]
+    layers = net.layers_f64()
+    # sample: pairs of a 600-particle box at the same density (same per-pair work)
+    from nepx.configs import make_system
+    small = make_system(shape_name, 600, 0.45, seed=123)
+    pairs = O.brute_pairs(small.x, small.L, sh["cutoff"])
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    done = 0
+    chunk = 1000
+    order = rng.permutation(len(pairs))
+    while time.perf_counter() - t0 < target_s and done < len(pairs):
+        pc = pairs[order[done:done + chunk]]
+        clouds, t = O.pair_clouds(small.x, small.q, pc, sh["sites"], small.L)
+        O.fd_pair_forces(clouds, len(sh["sites"]), t, sh["nep_cutoff"], hyp, layers, "tanh", 0.0, 1.0, 1e-5, 1e-5)
+        done += len(pc)
+    t_pairs = time.perf_counter() - t0
+    per_pair = t_pairs / done
+    # integrator per particle (vectorised FP64 numpy)
+    th = dict(xi=0.0, eta=0.0, xi_r=0.0, eta_r=0.0, theta=1.0, e_r=1.0)
+    st = dict(x=spec.x, v=spec.v, q=spec.q, p=spec.p, m=spec.mass)
+    t1 = time.perf_counter()
+    O.nvt_step(st, np.zeros_like(spec.x), np.zeros_like(spec.x), spec.dt, spec.kT, spec.tau, spec.L, spec.moi, th)
+    per_particle = (time.perf_counter() - t1) / spec.n
+    # pairs per particle of the full workload (half list) at this density
+    p_per_particle = len(pairs) / small.n
+    step_time = spec.n * (p_per_particle * per_pair + per_particle)
+    return {"value": spec.n / step_time, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{done} pairs of a 600-tetrahedron box at the same density through the reference "
+                      f"algorithm (10-row forward-FD stencil, NEP descriptor, FFNN, clip; FP64 numpy, 1 thread) "
+                      f"in {t_pairs:.1f} s, scaled by pairs per particle ({p_per_particle:.1f}); FP64 NVT "
+                      f"update timed on the full state"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="tet32k")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph", type=int, default=1, help="capture the step in a hipGraph (1) or launch eagerly (0)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "nepx runs on one GPU per process; multi-GPU runs are not part of this build "
+                                   "(independent systems shard by placing whole systems per GPU)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import nepx
+    from nepx.configs import WORKLOADS, make_system
+    from nepx.engine import StepRunner
+
+    w = WORKLOADS[a.workload]
+    shape = nepx.get_shape(w["shape"])
+    hyp = shape.hypers
+    D = nepx.n_descriptor(*hyp)
+    widths = [64, 64] if D == 40 else [128, 128]
+    spec = make_system(w["shape"], w["n"], w["phi"], kT=w["kT"], dt=w["dt"], seed=2024)
+    dev = torch.device("cuda:0")
+    net = nepx.EnergyNet.random(D, widths, seed=7, device=dev)
+    ar = nepx.Arena([spec], shape, hyp, net, device=dev)
+    runner = StepRunner(ar, use_graph=bool(a.graph))
+    runner.init()
+    runner.run(a.warmup)
+    torch.cuda.synchronize()
+    ar.check()
+
+    # timed region: K steps, events around every fused pair kernel on the step stream
+    stream = runner.stream
+    fused_ev = runner.enable_kernel_events(a.steps)
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    runner.run(a.steps)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    flags = ar.check()
+    ms = t_start.elapsed_time(t_end)
+    ms_step = ms / a.steps
+    fused_ms = runner.kernel_event_ms()
+    P = int(flags[3])
+    th = ar.thermo()[0]
+
+    flops = ffnn_flops_per_pair(D, widths) * P
+    achieved = flops / (fused_ms * 1e-3) / 1e12
+    res = {
+        "metric": "particle-steps/s at 32k tetrahedra",
+        "value": spec.n / (ms_step * 1e-3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair math (MFMA f32), f64 integrator",
+        "data": "synthetic (seeded lattice + random orientations, random-init FFNN)",
+        "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc=2.5 hypers={hyp} D={D} "
+                               f"FFNN {D}->{widths[0]}->{widths[1]}->1 tanh, NVT kT={w['kT']} dt={w['dt']}",
+                   "particles": spec.n, "pairs_per_step": P, "hipgraph": bool(a.graph),
+                   "list_builds": int(flags[4]), "wall_s": wall},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+                     "kernel": "k_fused (descriptor + FFNN fwd/bwd + adjoint)", "kernel_ms": fused_ms,
+                     "flops_per_launch": flops},
+        "thermo": {"Kt_per_dof": float(th["Kt"] / spec.ndof * 2), "U": float(th["U"])},
+    }
+    if not a.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(spec, w["shape"], hyp, net)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

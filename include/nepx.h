@@ -1,0 +1,184 @@
+/*
+ * nepx -- MI355X-native rigid-body NEP pair step (C ABI).
+ *
+ * One shared library, libnepx.so, built for gfx950 with hipcc.  Every entry
+ * point is asynchronous on the HIP stream passed in (NULL = the default
+ * stream), takes caller-owned device buffers as plain pointers, returns 0 on
+ * success or a negative NEPX_E* code, and never throws.  nepx_last_error()
+ * returns the message of the last failure on the calling host thread.
+ *
+ * The reference has no C FFI: its boundary is a set of Python class methods
+ * that launch CuPy raw kernels.  Each entry below names the reference
+ * function (file:line, paths relative to the reference checkout) whose work
+ * it replaces; the Python package anisotropic-ml_amd/nepx keeps the
+ * reference method names on top of these calls.
+ */
+#ifndef NEPX_H
+#define NEPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEPX_OK 0
+#define NEPX_EINVAL (-1)    /* bad argument (shape, size, null pointer)        */
+#define NEPX_EUNSUP (-2)    /* no kernel instance for these hypers / widths    */
+#define NEPX_EHIP (-3)      /* a HIP runtime call failed                       */
+#define NEPX_ECAP (-4)      /* a size exceeds a documented maximum             */
+
+#define NEPX_MAX_SITES 16   /* body-frame sites per particle                   */
+#define NEPX_ACT_TANH 0
+#define NEPX_ACT_RELU 1
+
+/* Pair model: rigid-body sites + NEP hypers (DescriptorGenerator.py:37-43,58-125). */
+typedef struct nepx_pair_model {
+    int32_t n_sites;                 /* Ns sites per body; a pair cloud has 2*Ns */
+    int32_t nrad, nang, lmax;        /* D = (nrad+1) + (nang+1)*lmax             */
+    float r_nep;                     /* descriptor cutoff (nep_cutoff)           */
+    float sites[3 * NEPX_MAX_SITES]; /* body-frame sites, xyz interleaved        */
+} nepx_pair_model;
+
+/* Energy network (ForceEvaluator.py:6-26): D -> width (-> width)*n_hidden -> 1.
+ * w_packed is a device buffer written by nepx_net_pack(). */
+typedef struct nepx_net {
+    int32_t d_in, width, n_hidden, act;
+    float en_min, en_max;            /* E = en_min + (en_max-en_min) * NN(G)    */
+    const float* w_packed;
+} nepx_net;
+
+/* Per-system parameters (device array of n_sys records, 16 doubles + 16 ints). */
+typedef struct nepx_sysparam {
+    double L, kT, dt, tau, ndof, rndof;  /* box, temperature, step, NH tau, N_dof, RN_dof */
+    double moi[3];                       /* diagonal inertia (Sim.py:43-44)      */
+    double r_list, rc, skin;             /* list radius, COM cutoff, skin        */
+    double pad[4];
+    int32_t p_start, p_end;              /* particle range (block aligned start) */
+    int32_t blk_start, blk_end;          /* integrator block range               */
+    int32_t nc, cell_off;                /* cells per side (0 = all pairs), offset */
+    int32_t ipad[10];
+} nepx_sysparam;
+
+/* Thermostat record per system (Sim.py:99-111, IntegrateStepTwoKernels.py:74-130). */
+typedef struct nepx_thermo {
+    double xi, eta, xi_r, eta_r;     /* Nose-Hoover variables                    */
+    double theta, e_r;               /* exp(-xi dt/2), exp(-xi_r dt/2)            */
+    double Kt, Kr, U;                /* kinetic (trans, rot) and potential energy */
+    double max_disp2;                /* largest squared displacement since build */
+    double pad[6];
+} nepx_thermo;
+
+/* Particle arena, SoA, device pointers (Sim.py:147-169 holds the same state AoS f32). */
+typedef struct nepx_state {
+    int32_t n;                       /* particles incl. padding (mass 0)          */
+    int32_t n_sys;
+    double *x, *y, *z, *vx, *vy, *vz;
+    double *q0, *q1, *q2, *q3;       /* orientation, scalar first                */
+    double *p0, *p1, *p2, *p3;       /* angular-momentum quaternion (angmom)     */
+    double *mass;
+    double *fx, *fy, *fz, *tx, *ty, *tz, *upot;
+    double *xb, *yb, *zb;            /* positions at the last list build          */
+    float* qf;                       /* float4 copy of q read by the pair kernels */
+    int32_t* sys;                    /* system of each particle                   */
+    nepx_sysparam* sp;               /* n_sys records                             */
+    nepx_thermo* th;                 /* n_sys records                             */
+    double* partials;                /* 4 doubles per integrator block            */
+} nepx_state;
+
+/* Verlet half list (i<j, rows sorted by j) + reverse rows + per-step compaction. */
+typedef struct nepx_nlist {
+    int64_t cap;                     /* slots in nbr/slot_row                     */
+    int32_t n_cells_total;
+    int32_t* cell_count;             /* n_cells_total+1 */
+    int32_t* cell_start;             /* n_cells_total+1 */
+    int32_t* cell_fill;              /* n_cells_total   */
+    int32_t* cell_list;              /* n               */
+    int32_t* cell_of;                /* n               */
+    int32_t* row_count;              /* n+1 (forward), then scanned to row_start  */
+    int32_t* row_start;              /* n+1 */
+    int32_t* rrow_count;             /* n+1 */
+    int32_t* rrow_start;             /* n+1 */
+    int32_t* nbr;                    /* cap: j of slot s                          */
+    int32_t* slot_row;               /* cap: i of slot s                          */
+    int32_t* rnbr;                   /* cap: reverse rows, then slot ids          */
+    int32_t* cidx;                   /* cap: compacted index of slot or -1        */
+    int32_t* blk_count;              /* compaction partials (cap/1024+1)           */
+    int32_t* flags;                  /* [0] rebuild request, [1] overflow, [2] P_L, [3] P, [4] builds */
+    int32_t* pair_i;                 /* cap: compacted pairs                       */
+    int32_t* pair_j;
+    float* pair_t;                   /* cap*4: minimum-image t = X_j - X_i (f32)  */
+} nepx_nlist;
+
+/* ---- library ---------------------------------------------------------- */
+const char* nepx_last_error(void);
+int nepx_version(void);
+/* Sizes of the shared structs, for binding checks from the host language. */
+int nepx_struct_sizes(int32_t* out4);
+
+/* ---- network ----------------------------------------------------------- */
+/* Bytes of the packed weight buffer for these widths. */
+int64_t nepx_net_packed_floats(int32_t d_in, int32_t width, int32_t n_hidden);
+/* Pack host float32 weights (PyTorch nn.Linear layout, W[out][in]) into the
+ * MFMA operand order.  Replaces TheEnergyNet.load_state_dict + .to(cuda)
+ * (ForceEvaluator.py:55-61).  ws/bs: n_hidden+2 host pointers each. */
+int nepx_net_pack(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
+                  const float* const* bs, float* host_out);
+
+/* ---- descriptor, energy network, adjoint ------------------------------- */
+/* G (P x D, f32) of compacted pairs.  Replaces pos_to_pts + pts_to_nep_descriptors
+ * (DescriptorGenerator.py:163-218,296-432); one row per pair (no FD stencil). */
+int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
+                        const int32_t* pair_j, const float* pair_t, const int32_t* n_pairs_dev,
+                        int64_t max_pairs, float* G, void* stream);
+/* E (P) and dE/dG (P x D) by MFMA, f32 in / f32 accumulate.  Replaces
+ * TheEnergyNet.forward + the forward-difference gradient (ForceEvaluator.py:76-113). */
+int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int32_t* n_pairs_dev,
+                      int64_t max_pairs, float* E, float* dEdG, void* stream);
+/* Hand-written adjoint: dE/dG -> per-pair (F_i, tau_i, tau_j), each clipped to
+ * +-clip when clip > 0.  Replaces evaluate_gradients/evaluate_forces_torques
+ * (ForceEvaluator.py:76-132).  out: P x 12 floats = {F,E}{tau_i,0}{tau_j,0}. */
+int nepx_descriptor_adjoint(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
+                            const int32_t* pair_j, const float* pair_t, const int32_t* n_pairs_dev,
+                            int64_t max_pairs, const float* dEdG, const float* E, float clip,
+                            float* out, void* stream);
+/* Fused descriptor -> MFMA network -> adjoint for the step (no G in HBM). */
+int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
+                    const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                    const int32_t* n_pairs_dev, int64_t max_pairs, float clip, float* out,
+                    void* stream);
+
+/* ---- neighbour list ---------------------------------------------------- */
+/* Rebuild the half list within r_list of each system when flags[0] != 0
+ * (always when force != 0).  Replaces the host cKDTree query
+ * (Sim.py:85-92,192-198). */
+int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, void* stream);
+/* Load an explicit (M,2) candidate list (i<j not required) into the list rows.
+ * Replaces passing Nlist to generate_nep_descriptors (DescriptorGenerator.py:136-185). */
+int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64_t* pairs, int64_t m,
+                          void* stream);
+/* Per-step COM filter d2 < rc^2 (FP64) and compaction into pair_i/pair_j/pair_t;
+ * count in flags[3].  Replaces DescriptorGenerator.py:168-185. */
+int nepx_pairs_filter(nepx_state* st, nepx_nlist* nl, void* stream);
+/* Deterministic gather of per-pair outputs into F, T, upot (FP64).
+ * Replaces the index_add_ scatter (ForceEvaluator.py:135-142). */
+int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* pair_out, void* stream);
+
+/* ---- integrator -------------------------------------------------------- */
+/* One NVT step given F, T: integrate_step_two + integrate_step_one
+ * (NVT.py:17-129) in FP64, then per-system kinetic energies, thermostat
+ * update for the next step (update_variables) and the list rebuild test. */
+int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream);
+/* Kinetic energies of the current state and the thermostat update that the
+ * reference performs at the start of the first step (Sim.py:147-169). */
+int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, void* stream);
+
+/* One full MD step: [list rebuild if needed] -> filter -> fused pair kernel ->
+ * gather -> NVT.  Replaces Sim.step (Sim.py:257-285).  pair_out: cap x 12 floats. */
+int nepx_md_step(nepx_state* st, nepx_nlist* nl, const nepx_pair_model* m, const nepx_net* net,
+                 float clip, float* pair_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEPX_H */

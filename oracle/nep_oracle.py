@@ -1,0 +1,586 @@
+"""FP64 host oracle for the nep_simulator MD step -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker, not the product.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it.  The product path (``anisotropic-ml_amd/nepx``) never imports it and
+fails loudly when its HIP library is missing.
+
+It restates the reference algorithm in float64 numpy, written from the
+reference source text (never imported):
+
+* site placement            nep_simulator/cudakernels/PosToPtsKernels.py:3-150
+* minimum image + COM cut   nep_simulator/DescriptorGenerator.py:163-185
+* radial descriptor         nep_simulator/cudakernels/NepDescriptorKernels.py:3-93
+* angular descriptor        nep_simulator/cudakernels/NepDescriptorKernels.py:96-144,
+                            nep_simulator/DescriptorGenerator.py:368-432,
+                            nep_simulator/cudakernels/SingleKernelDebug.py:5-86
+* FFNN                      nep_simulator/ForceEvaluator.py:6-26
+* energy scaling, FD forces nep_simulator/ForceEvaluator.py:76-146 (+ 04/FFNN.py:67-71)
+* FD stencil                nep_simulator/cudakernels/PosToPtsKernels.py:153-446
+* NVT integrator            nep_simulator/NVT.py:17-129,
+                            nep_simulator/cudakernels/IntegrateStepOneKernels.py:3-251,
+                            nep_simulator/cudakernels/IntegrateStepTwoKernels.py:2-130
+
+Pinning (each function is checked in tests/test_oracle.py against something
+other than its own formula): scipy Rotation (placement), scipy eval_chebyt /
+eval_legendre (descriptor bases), the spherical-harmonic addition theorem
+(angular descriptor), brute force vs scipy cKDTree (pair sets), torch
+nn.Linear in float64 (FFNN), torch autograd + central finite differences
+(adjoint forces and torques), closed-form free-particle / spherical-top
+motion and time reversibility (integrator).  No reference-held golden
+vectors exist for this path (the reference has no tests).
+"""
+import math
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# Body-frame interaction sites
+# ---------------------------------------------------------------------------
+
+
+def _tet10_sites(edge=1.0):
+    a = edge / (2.0 * math.sqrt(2.0))
+    v = np.array([[a, a, a], [a, -a, -a], [-a, a, -a], [-a, -a, a]])
+    mids = []
+    for i in range(4):
+        for j in range(i + 1, 4):
+            mids.append(0.5 * (v[i] + v[j]))
+    return np.vstack([v, np.array(mids)])
+
+
+def _cyl12_sites(length=3.0, radius=0.5, n=12, twist_deg=90.0):
+    # helix template after 05/PtsPlacement.py:35-62 (pts_along_axis):
+    # n sites at constant radius, azimuth spread over the total twist,
+    # evenly spaced along the axis, centroid removed.
+    ang = np.linspace(0.0, np.deg2rad(twist_deg), n)
+    s = np.zeros((n, 3))
+    s[:, 0] = radius * np.cos(ang)
+    s[:, 1] = radius * np.sin(ang)
+    s[:, 2] = np.linspace(-0.5 * length, 0.5 * length, n)
+    return s - s.mean(axis=0)
+
+
+def shape_table():
+    """Site tables and per-shape constants.
+
+    cube_v2 / th_v1 / pbp_v3 follow nep_simulator/DescriptorGenerator.py:63-118
+    (sites, en_min, en_max, COM cutoff, nep cutoff).  cube8 / tet10 / cyl12 are
+    the benchmark shapes (unit mass, solid-body inertia, rc = r_nep).
+    """
+    cube_v2 = 1.7 * np.array([[1.0, 0, 0], [-1.0, 0, 0], [0, 1.0, 0],
+                              [0, -1.0, 0], [0, 0, 1.0], [0, 0, -1.0]])
+    th_v1 = np.array([[-0.45, -0.45, -0.45], [-0.45, 0.45, 0.45],
+                      [0.45, -0.45, 0.45], [0.45, 0.45, -0.45]])
+    pbp_v3 = np.array([[-1.46946, 2.02254, 0.0], [-2.37764, -0.77254, 0.0],
+                       [-0.0, -2.5, 0.0], [2.37764, -0.77254, 0.0],
+                       [1.46946, 2.02254, 0.0], [0.0, 0.0, 3.5],
+                       [0.0, 0.0, -3.5]])
+    cube8 = 0.5 * np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1)
+                            for sz in (-1, 1)], dtype=np.float64)
+    return {
+        "cube_v2": dict(sites=cube_v2, en_min=-5.1, en_max=15.0, cutoff=5.80, nep_cutoff=4.5),
+        "th_v1": dict(sites=th_v1, en_min=-6.7, en_max=15.0, cutoff=5.75, nep_cutoff=4.5),
+        "pbp_v3": dict(sites=pbp_v3, en_min=-2.5, en_max=15.0, cutoff=8.1, nep_cutoff=7.5),
+        "cube8": dict(sites=cube8, en_min=0.0, en_max=1.0, cutoff=2.5, nep_cutoff=2.5),
+        "tet10": dict(sites=_tet10_sites(), en_min=0.0, en_max=1.0, cutoff=2.5, nep_cutoff=2.5),
+        "cyl12": dict(sites=_cyl12_sites(), en_min=0.0, en_max=1.0, cutoff=4.0, nep_cutoff=4.0),
+    }
+
+
+# ---------------------------------------------------------------------------
+# Quaternions (scalar first)
+# ---------------------------------------------------------------------------
+
+
+def quat_rotate(q, v):
+    """R(q) v = (w^2-|u|^2) v + 2 w (u x v) + 2 (u.v) u.
+
+    nep_simulator/cudakernels/PosToPtsKernels.py:54-71.  q (...,4), v (...,3).
+    """
+    q = np.asarray(q, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    w = q[..., :1]
+    u = q[..., 1:]
+    a = w * w - np.sum(u * u, axis=-1, keepdims=True)
+    return a * v + 2.0 * w * np.cross(u, v) + 2.0 * np.sum(u * v, axis=-1, keepdims=True) * u
+
+
+def quat_mul(a, b):
+    """Hamilton product, nep_simulator/MathUtils.py:3-24."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    s = a[..., 0] * b[..., 0] - np.sum(a[..., 1:] * b[..., 1:], axis=-1)
+    v = a[..., :1] * b[..., 1:] + b[..., :1] * a[..., 1:] + np.cross(a[..., 1:], b[..., 1:])
+    return np.concatenate([s[..., None], v], axis=-1)
+
+
+def quat_conj(q):
+    q = np.array(q, dtype=np.float64, copy=True)
+    q[..., 1:] *= -1.0
+    return q
+
+
+# ---------------------------------------------------------------------------
+# Pairs
+# ---------------------------------------------------------------------------
+
+
+def min_image(t, L):
+    """Single wrap per component, nep_simulator/DescriptorGenerator.py:168-170."""
+    t = np.where(t > 0.5 * L, t - L, t)
+    t = np.where(t < -0.5 * L, L + t, t)
+    return t
+
+
+def pair_d2(X, i, j, L):
+    """FP64 squared distance with the single-wrap minimum image.
+
+    The sum order (x*x + y*y) + z*z is fixed so every implementation of the
+    predicate d2 < r*r agrees bit for bit.
+    """
+    t = min_image(X[j] - X[i], L)
+    return (t[:, 0] * t[:, 0] + t[:, 1] * t[:, 1]) + t[:, 2] * t[:, 2]
+
+
+def brute_pairs(X, L, r):
+    """All i<j with d2 < r*r (FP64 predicate), sorted by (i, j).
+
+    Replaces the host KD-tree list plus per-step COM filter
+    (nep_simulator/Sim.py:85-92, DescriptorGenerator.py:168-175) by the exact
+    set that filter selects.
+    """
+    X = np.asarray(X, dtype=np.float64)
+    N = len(X)
+    out = []
+    r2 = float(r) * float(r)
+    for i in range(N - 1):
+        j = np.arange(i + 1, N)
+        d2 = pair_d2(X, np.full_like(j, i), j, L)
+        sel = j[d2 < r2]
+        if len(sel):
+            out.append(np.stack([np.full_like(sel, i), sel], axis=1))
+    if not out:
+        return np.zeros((0, 2), dtype=np.int64)
+    return np.concatenate(out, axis=0).astype(np.int64)
+
+
+def pair_clouds(X, Q, pairs, sites, L):
+    """Site cloud of each pair in the frame centred on i (App. A.1).
+
+    p_a = R(q_i) b_a (a < Ns), p_{Ns+b} = R(q_j) b_b + t,  t = mi(X_j - X_i).
+    nep_simulator/cudakernels/PosToPtsKernels.py:3-79,
+    nep_simulator/DescriptorGenerator.py:168-204.
+    Returns clouds (P, 2Ns, 3) and t (P, 3).
+    """
+    X = np.asarray(X, dtype=np.float64)
+    Q = np.asarray(Q, dtype=np.float64)
+    i = pairs[:, 0]
+    j = pairs[:, 1]
+    t = min_image(X[j] - X[i], L)
+    b = np.asarray(sites, dtype=np.float64)
+    pi = quat_rotate(Q[i][:, None, :], b[None, :, :])
+    pj = quat_rotate(Q[j][:, None, :], b[None, :, :]) + t[:, None, :]
+    return np.concatenate([pi, pj], axis=1), t
+
+
+# ---------------------------------------------------------------------------
+# Descriptor
+# ---------------------------------------------------------------------------
+
+
+def _cheb_basis(rho, nmax):
+    """f_n = (T_n(x)+1)/2 * fc, x = 2(rho-1)^2-1, fc = (1+cos(pi rho))/2, n=0..nmax.
+
+    No truncation beyond rho = 1 (NepDescriptorKernels.py:52-74).
+    Returns f (nmax+1, ...) and df/drho.
+    """
+    x = 2.0 * (rho - 1.0) ** 2 - 1.0
+    dx = 4.0 * (rho - 1.0)
+    fc = 0.5 * (1.0 + np.cos(np.pi * rho))
+    dfc = -0.5 * np.pi * np.sin(np.pi * rho)
+    T = [np.ones_like(x), x]
+    dT = [np.zeros_like(x), np.ones_like(x)]
+    for n in range(1, nmax):
+        T.append(2.0 * x * T[n] - T[n - 1])
+        dT.append(2.0 * T[n] + 2.0 * x * dT[n] - dT[n - 1])
+    T = np.stack(T[: nmax + 1])
+    dT = np.stack(dT[: nmax + 1])
+    f = 0.5 * (T + 1.0) * fc
+    df = 0.5 * dT * dx * fc + 0.5 * (T + 1.0) * dfc
+    return f, df
+
+
+def _legendre(c, lmax):
+    """P_0..P_lmax and derivatives by the three-term recurrence (NepDescriptorKernels.py:126-140)."""
+    P = [np.ones_like(c), c]
+    dP = [np.zeros_like(c), np.ones_like(c)]
+    for l in range(1, lmax):
+        P.append(((2 * l + 1) * c * P[l] - l * P[l - 1]) / (l + 1))
+        dP.append(dP[l - 1] + (2 * l + 1) * P[l])
+    return np.stack(P[: lmax + 1]), np.stack(dP[: lmax + 1])
+
+
+def n_descriptor(nrad, nang, lmax):
+    """D = (nrad+1) + (nang+1)*lmax, nep_simulator/DescriptorGenerator.py:37-42."""
+    return (nrad + 1) + (nang + 1) * lmax
+
+
+def descriptor(clouds, r_nep, nrad, nang, lmax):
+    """NEP descriptor of each pair cloud, direct (all a, b) form (App. A.2).
+
+    g_rad[n] = sum_a f_n(a)                                  n = 0..nrad
+    g_ang[n*lmax + l-1] = sum_a sum_b f_n(a) f_n(b) P_l(cos_ab)  n = 0..nang, l = 1..lmax
+    The angular double sum includes a == b and both orders, as the live
+    path does (DescriptorGenerator.py:376-411, SingleKernelDebug.py:23-73).
+    clouds (P, Nd, 3) float64 -> G (P, D) float64.
+    """
+    assert nang <= nrad
+    c = clouds - clouds.mean(axis=1, keepdims=True)
+    r = np.linalg.norm(c, axis=-1)                      # (P, Nd)
+    u = c / r[..., None]
+    f, _ = _cheb_basis(r / r_nep, nrad)                 # (nrad+1, P, Nd)
+    g_rad = f.sum(axis=-1).T                            # (P, nrad+1)
+    cos = np.einsum("pak,pbk->pab", u, u)
+    cos = np.clip(cos, -1.0, 1.0)
+    P, _ = _legendre(cos, lmax)                         # (lmax+1, P, Nd, Nd)
+    fa = f[: nang + 1]                                  # (nang+1, P, Nd)
+    g_ang = np.empty((clouds.shape[0], nang + 1, lmax))
+    for l in range(1, lmax + 1):
+        # sum_ab f_n(a) P_l(ab) f_n(b)
+        g_ang[:, :, l - 1] = np.einsum("npa,pab,npb->pn", fa, P[l], fa)
+    return np.concatenate([g_rad, g_ang.reshape(clouds.shape[0], -1)], axis=1)
+
+
+def descriptor_grad(clouds, dG, r_nep, nrad, nang, lmax):
+    """Adjoint of ``descriptor``: dE/dp (P, Nd, 3) given dE/dG (P, D).
+
+    Differentiates the direct Legendre form (not the harmonic form the HIP
+    kernel uses), including the centroid coupling
+    dE/dp_a = dE/dr_a - (1/Nd) sum_b dE/dr_b.
+    """
+    Pn, Nd, _ = clouds.shape
+    c = clouds - clouds.mean(axis=1, keepdims=True)
+    r = np.linalg.norm(c, axis=-1)
+    u = c / r[..., None]
+    rho = r / r_nep
+    f, df = _cheb_basis(rho, nrad)                      # (nrad+1, P, Nd)
+    cos = np.clip(np.einsum("pak,pbk->pab", u, u), -1.0, 1.0)
+    Pl, dPl = _legendre(cos, lmax)
+    dr = dG[:, : nrad + 1]                               # (P, nrad+1)
+    da = dG[:, nrad + 1:].reshape(Pn, nang + 1, lmax)    # (P, nang+1, lmax)
+    # dE/drho_a and dE/dcos_ab
+    dE_df = np.einsum("pn,npa->npa", dr, np.ones_like(f))  # (nrad+1, P, Nd)
+    dE_dcos = np.zeros_like(cos)
+    for l in range(1, lmax + 1):
+        w = da[:, :, l - 1]                              # (P, nang+1)
+        # d/df_n(a) of sum_ab f f P = 2 sum_b P_l(ab) f_n(b)
+        dE_df[: nang + 1] += 2.0 * np.einsum("pn,pab,npb->npa", w, Pl[l], f[: nang + 1])
+        dE_dcos += np.einsum("pn,npa,npb->pab", w, f[: nang + 1], f[: nang + 1]) * dPl[l]
+    dE_drho = np.sum(dE_df * df, axis=0)                # (P, Nd)
+    g = (dE_drho / r_nep)[..., None] * u
+    # cos_ab = u_a.u_b ; d cos_ab / d r_a = (u_b - cos_ab u_a)/|r_a|
+    # dE_dcos is symmetric; each unordered (a,b) appears twice in sum_ab
+    S = dE_dcos + np.transpose(dE_dcos, (0, 2, 1))
+    term = np.einsum("pab,pbk->pak", S, u) - np.sum(S * cos, axis=2)[..., None] * u
+    g = g + term / r[..., None]
+    return g - g.mean(axis=1, keepdims=True)
+
+
+# ---------------------------------------------------------------------------
+# Spherical-harmonic form (independent restatement used to pin the angular term)
+# ---------------------------------------------------------------------------
+
+
+def real_harmonics_normalised(u, lmax):
+    """Real harmonics Yhat_lm with sum_m Yhat_lm(u) Yhat_lm(v) = P_l(u.v).
+
+    Built from scipy's complex sph_harm as an independent check of the HIP
+    kernel's Cartesian recurrence.  u (..., 3) unit vectors.
+    Returns list over l=0..lmax of arrays (..., 2l+1).
+    """
+    from scipy.special import sph_harm_y  # scipy >= 1.15
+    theta = np.arccos(np.clip(u[..., 2], -1.0, 1.0))
+    phi = np.arctan2(u[..., 1], u[..., 0])
+    out = []
+    for l in range(lmax + 1):
+        comps = []
+        for m in range(-l, l + 1):
+            Y = sph_harm_y(l, abs(m), theta, phi)
+            if m < 0:
+                comps.append(math.sqrt(2.0) * Y.imag)
+            elif m == 0:
+                comps.append(Y.real)
+            else:
+                comps.append(math.sqrt(2.0) * Y.real)
+        out.append(np.stack(comps, axis=-1) * math.sqrt(4.0 * math.pi / (2 * l + 1)))
+    return out
+
+
+def descriptor_harmonic(clouds, r_nep, nrad, nang, lmax):
+    """Angular term by the addition theorem: sum_ab f f P_l = sum_m (sum_a f Yhat_lm)^2."""
+    c = clouds - clouds.mean(axis=1, keepdims=True)
+    r = np.linalg.norm(c, axis=-1)
+    u = c / r[..., None]
+    f, _ = _cheb_basis(r / r_nep, nrad)
+    g_rad = f.sum(axis=-1).T
+    Y = real_harmonics_normalised(u, lmax)
+    g_ang = np.empty((clouds.shape[0], nang + 1, lmax))
+    for l in range(1, lmax + 1):
+        A = np.einsum("npa,pam->pnm", f[: nang + 1], Y[l])
+        g_ang[:, :, l - 1] = np.sum(A * A, axis=-1)
+    return np.concatenate([g_rad, g_ang.reshape(clouds.shape[0], -1)], axis=1)
+
+
+# ---------------------------------------------------------------------------
+# FFNN (TheEnergyNet, nep_simulator/ForceEvaluator.py:6-26)
+# ---------------------------------------------------------------------------
+
+
+def _act(z, act):
+    if act == "tanh":
+        a = np.tanh(z)
+        return a, 1.0 - a * a
+    if act == "relu":
+        return np.maximum(z, 0.0), (z > 0).astype(z.dtype)
+    raise ValueError(act)
+
+
+def ffnn_forward(G, layers, act="tanh"):
+    """layers = [(W, b), ...] with W (out, in); activation after every layer but the last."""
+    h = np.asarray(G, dtype=np.float64)
+    for W, b in layers[:-1]:
+        h, _ = _act(h @ W.T + b, act)
+    W, b = layers[-1]
+    return (h @ W.T + b)[:, 0]
+
+
+def ffnn_forward_backward(G, layers, act="tanh"):
+    """Returns (y (P,), dy/dG (P, D))."""
+    h = np.asarray(G, dtype=np.float64)
+    ders = []
+    for W, b in layers[:-1]:
+        h, d = _act(h @ W.T + b, act)
+        ders.append(d)
+    W, b = layers[-1]
+    y = (h @ W.T + b)[:, 0]
+    g = np.broadcast_to(W[0], h.shape).copy()
+    for (W, b), d in zip(reversed(layers[:-1]), reversed(ders)):
+        g = (g * d) @ W
+    return y, g
+
+
+def random_layers(D, widths, seed, std_scale=1.0):
+    """Seeded N(0, 1/fan_in) weights, zero biases; widths e.g. [64, 64] -> D->64->64->1."""
+    rng = np.random.default_rng(seed)
+    dims = [D] + list(widths) + [1]
+    out = []
+    for a, b in zip(dims[:-1], dims[1:]):
+        W = rng.standard_normal((b, a)) * (std_scale / math.sqrt(a))
+        out.append((W, np.zeros(b)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Pair energies, analytic forces and torques
+# ---------------------------------------------------------------------------
+
+
+def pair_forces(X, Q, pairs, shape, L, hyp, layers, act="tanh", en_min=0.0, en_max=1.0, r_nep=None):
+    """Per-pair energy and analytic (F_i, tau_i, tau_j), FP64.
+
+    E_ij = en_min + (en_max-en_min) NN(G_ij)   (ForceEvaluator.py:119-121, 04/FFNN.py:67-71)
+    F_i  = -sum_{a in i} dE/dp_a,  F_j = -F_i (ForceEvaluator.py:138-139)
+    tau_i = sum_{a in i} (p_a - X_i) x (-dE/dp_a)  (lab axes through X_i,
+    the limit of PosToPtsKernels.py:225-318), tau_j likewise about X_j.
+    """
+    nrad, nang, lmax = hyp
+    sites = shape["sites"]
+    Ns = len(sites)
+    rn = shape["nep_cutoff"] if r_nep is None else r_nep
+    clouds, t = pair_clouds(X, Q, pairs, sites, L)
+    G = descriptor(clouds, rn, nrad, nang, lmax)
+    y, dy = ffnn_forward_backward(G, layers, act)
+    scale = en_max - en_min
+    E = en_min + scale * y
+    dp = scale * descriptor_grad(clouds, dy, rn, nrad, nang, lmax)   # dE/dp (P, Nd, 3)
+    Fi = -dp[:, :Ns].sum(axis=1)
+    arm_i = clouds[:, :Ns]
+    arm_j = clouds[:, Ns:] - t[:, None, :]
+    Ti = np.cross(arm_i, -dp[:, :Ns]).sum(axis=1)
+    Tj = np.cross(arm_j, -dp[:, Ns:]).sum(axis=1)
+    return dict(G=G, E=E, F=Fi, Ti=Ti, Tj=Tj, t=t)
+
+
+def scatter_forces(N, pairs, Fi, Ti, Tj):
+    """F[i] += f, F[j] -= f, T[i] += tau_i, T[j] += tau_j (ForceEvaluator.py:135-142), FP64."""
+    F = np.zeros((N, 3))
+    T = np.zeros((N, 3))
+    np.add.at(F, pairs[:, 0], Fi)
+    np.add.at(F, pairs[:, 1], -Fi)
+    np.add.at(T, pairs[:, 0], Ti)
+    np.add.at(T, pairs[:, 1], Tj)
+    return F, T
+
+
+def total_energy(X, Q, shape, L, hyp, layers, act="tanh", en_min=0.0, en_max=1.0, rc=None):
+    rc = shape["cutoff"] if rc is None else rc
+    pairs = brute_pairs(X, L, rc)
+    if len(pairs) == 0:
+        return 0.0
+    clouds, _ = pair_clouds(X, Q, pairs, shape["sites"], L)
+    G = descriptor(clouds, shape["nep_cutoff"], *hyp)
+    return float(np.sum(en_min + (en_max - en_min) * ffnn_forward(G, layers, act)))
+
+
+def system_forces(X, Q, shape, L, hyp, layers, act="tanh", en_min=0.0, en_max=1.0, rc=None):
+    rc = shape["cutoff"] if rc is None else rc
+    pairs = brute_pairs(X, L, rc)
+    r = pair_forces(X, Q, pairs, shape, L, hyp, layers, act, en_min, en_max)
+    F, T = scatter_forces(len(X), pairs, r["F"], r["Ti"], r["Tj"])
+    return F, T, pairs, r
+
+
+# ---------------------------------------------------------------------------
+# Reference-style forward finite-difference stencil (compat mode)
+# ---------------------------------------------------------------------------
+
+
+def _axis_quat(axis, ang):
+    q = np.zeros(4)
+    q[0] = math.cos(0.5 * ang)
+    q[1 + axis] = math.sin(0.5 * ang)
+    return q
+
+
+def stencil_clouds(clouds, Ns, t, dx, dteta):
+    """The 10-row stencil of nep_simulator/DescriptorGenerator.py:187-255.
+
+    Rows 0..6 (frame of i): copy, i's sites + dx e_x/e_y/e_z, i's sites
+    rotated by dteta about x/y/z through the origin (= X_i)
+    (PosToPtsKernels.py:153-321).  Rows 7..9 (frame of j, get_pts_pairs2):
+    j's sites rotated by dteta about x/y/z through X_j (PosToPtsKernels.py:324-446).
+    The j-frame cloud lists j's sites first and i's sites shifted by -t.
+    Returns (10, P, Nd, 3).
+    """
+    rows = [clouds.copy()]
+    for ax in range(3):
+        c = clouds.copy()
+        c[:, :Ns, ax] += dx
+        rows.append(c)
+    for ax in range(3):
+        c = clouds.copy()
+        c[:, :Ns] = quat_rotate(_axis_quat(ax, dteta), c[:, :Ns])
+        rows.append(c)
+    jframe = np.concatenate([clouds[:, Ns:] - t[:, None, :], clouds[:, :Ns] - t[:, None, :]], axis=1)
+    for ax in range(3):
+        c = jframe.copy()
+        c[:, :Ns] = quat_rotate(_axis_quat(ax, dteta), c[:, :Ns])
+        rows.append(c)
+    return np.stack(rows)
+
+
+def fd_pair_forces(clouds, Ns, t, r_nep, hyp, layers, act, en_min, en_max, dx, dteta, clip=100.0):
+    """Reference FD forces: f = -(E_op - E_0)/dx scaled by the energy range, clipped at +-clip.
+
+    nep_simulator/ForceEvaluator.py:86-132 (forward differences, per-component clip).
+    """
+    st = stencil_clouds(clouds, Ns, t, dx, dteta)
+    Pn = clouds.shape[0]
+    G = descriptor(st.reshape(-1, *clouds.shape[1:]), r_nep, *hyp)
+    e = ffnn_forward(G, layers, act).reshape(10, Pn)
+    rng = en_max - en_min
+    F = -(e[1:4] - e[0]).T / dx * rng
+    Ti = -(e[4:7] - e[0]).T / dteta * rng
+    Tj = -(e[7:10] - e[0]).T / dteta * rng
+    return np.clip(F, -clip, clip), np.clip(Ti, -clip, clip), np.clip(Tj, -clip, clip)
+
+
+# ---------------------------------------------------------------------------
+# NVT (two-step Nose-Hoover translation + NO_SQUISH rotation)
+# ---------------------------------------------------------------------------
+
+
+def thermo_update(th, Kt, Kr, N_dof, RN_dof, dt, tau, kT):
+    """update_variables (IntegrateStepTwoKernels.py:93-125), FP64.
+
+    th = dict(xi, eta, xi_r, eta_r); returns new dict plus theta (translation
+    factor exp(-xi dt/2)) and e_r (rotation factor exp(-xi_r dt/2)).
+    """
+    c = 0.5 * (dt / tau) / tau
+    Tt = 2.0 * Kt / N_dof
+    xi_p = th["xi"] + c * (Tt / kT - 1.0)
+    xi = xi_p + c * (Tt / kT - 1.0)
+    eta = th["eta"] + xi_p * dt
+    Tr = 2.0 * Kr / RN_dof
+    xr_p = th["xi_r"] + c * (Tr / kT - 1.0)
+    xi_r = xr_p + c * (Tr / kT - 1.0)
+    eta_r = th["eta_r"] + xr_p * dt
+    return dict(xi=xi, eta=eta, xi_r=xi_r, eta_r=eta_r,
+                theta=math.exp(-0.5 * xi * dt), e_r=math.exp(-0.5 * dt * xi_r))
+
+
+def torque_dp(q, tau, dt):
+    """dp = dt * q (x) (0, R(q*) tau)  (IntegrateStepTwoKernels.py:22-58)."""
+    tb = quat_rotate(quat_conj(q), tau)
+    z = np.concatenate([np.zeros(tb.shape[:-1] + (1,)), tb], axis=-1)
+    return dt * quat_mul(q, z)
+
+
+def _perm(k, v):
+    """P_k v for the NO_SQUISH free-rotor sub-steps (IntegrateStepOneKernels.py:43-171)."""
+    w, x, y, z = v[..., 0], v[..., 1], v[..., 2], v[..., 3]
+    if k == 1:
+        return np.stack([-x, w, z, -y], axis=-1)
+    if k == 2:
+        return np.stack([-y, -z, w, x], axis=-1)
+    return np.stack([-z, y, -x, w], axis=-1)
+
+
+def no_squish(p, q, moi, dt):
+    """Free-rotor sequence 3(dt/2) 2(dt/2) 1(dt) 2(dt/2) 3(dt/2) (IntegrateStepOneKernels.py:43-171)."""
+    moi = np.broadcast_to(np.asarray(moi, dtype=np.float64), p.shape[:-1] + (3,))
+    for k, h in ((3, 0.5 * dt), (2, 0.5 * dt), (1, dt), (2, 0.5 * dt), (3, 0.5 * dt)):
+        pp = _perm(k, p)
+        qq = _perm(k, q)
+        phi = 0.25 / moi[..., k - 1] * np.sum(p * qq, axis=-1)
+        c = np.cos(h * phi)[..., None]
+        s = np.sin(h * phi)[..., None]
+        p, q = c * p + s * pp, c * q + s * qq
+    return p, q
+
+
+def rot_ke(q, p, moi):
+    """sum_k s_k^2/(2 I_k), s = 1/2 conj(q) (x) p (IntegrateStepOneKernels.py:231-248)."""
+    s = 0.5 * quat_mul(quat_conj(q), p)
+    moi = np.broadcast_to(np.asarray(moi, dtype=np.float64), q.shape[:-1] + (3,))
+    return 0.5 * np.sum(s[..., 1:] ** 2 / moi, axis=-1)
+
+
+def nvt_step(state, F, T, dt, kT, tau, L, moi, thermo):
+    """One integrate_step_two + integrate_step_one with given F, T (App. A.4).
+
+    state: dict x, v, q, p, m (FP64 arrays).  thermo: dict xi, eta, xi_r,
+    eta_r, theta, e_r already updated from the previous kinetic energies.
+    Returns (new_state, Kt, Kr) -- the kinetic energies feed the next
+    thermo_update.
+    """
+    x, v, q, p, m = (np.array(state[k], dtype=np.float64, copy=True) for k in "xvqpm")
+    th, er = thermo["theta"], thermo["e_r"]
+    a = F / m[:, None]
+    # step two: translation half kick and torque half kick
+    v = v * th + 0.5 * dt * a
+    dp = torque_dp(q, T, dt)
+    p = p * er + dp
+    # step one: translation (IntegrateStepOneKernels.py:202-207)
+    v = (v + 0.5 * dt * a) * th
+    x = x + v * dt
+    x = np.where(x > 0.5 * L, x - L, x)
+    x = np.where(x < -0.5 * L, x + L, x)
+    # step one: rotation
+    p = (p + dp) * er
+    p, q = no_squish(p, q, moi, dt)
+    q = q / np.linalg.norm(q, axis=-1, keepdims=True)
+    Kt = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
+    Kr = float(np.sum(rot_ke(q, p, moi)))
+    return dict(x=x, v=v, q=q, p=p, m=m), Kt, Kr
