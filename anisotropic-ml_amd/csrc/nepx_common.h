@@ -40,7 +40,11 @@ constexpr double csqrt(double x) {
     for (int i = 0; i < 64; ++i) r = 0.5 * (r + x / r);
     return r;
 }
-constexpr double cfact(int n) { return n <= 1 ? 1.0 : n * cfact(n - 1); }
+constexpr double cfact(int n) {
+    double r = 1.0;
+    for (int k = 2; k <= n; ++k) r *= k;
+    return r;
+}
 
 // ---- quaternion helpers (float, device) ------------------------------------
 // R(q) v = (w^2-|u|^2) v + 2 w (u x v) + 2 (u.v) u  as a 3x3 matrix, written

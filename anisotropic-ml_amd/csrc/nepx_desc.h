@@ -27,10 +27,20 @@ struct HarmTables {
     static constexpr int NLM = LMAX * (LMAX + 2);  // l = 1..LMAX
     static constexpr int NQ = (LMAX + 1) * (LMAX + 2) / 2;
     __host__ __device__ static constexpr int qi(int l, int m) { return l * (l + 1) / 2 + m; }
-    // K_lm = sqrt(2 (l-m)!/(l+m)!) for m > 0
-    __host__ __device__ static constexpr float kfac(int l, int m) {
-        return m == 0 ? 1.0f : (float)csqrt(2.0 * cfact(l - m) / cfact(l + m));
+};
+
+// K_lm = sqrt(2 (l-m)!/(l+m)!) for m > 0 (1 for m = 0), evaluated at compile
+// time; index l*(LMAX+1)+m.  Used through a constexpr local so every lookup
+// folds to an immediate once the (l, m) loops are unrolled.
+template <int LMAX>
+struct KTab {
+    float v[(LMAX + 1) * (LMAX + 1)];
+    constexpr KTab() : v() {
+        for (int l = 0; l <= LMAX; ++l)
+            for (int m = 0; m <= l; ++m)
+                v[l * (LMAX + 1) + m] = m == 0 ? 1.0f : (float)csqrt(2.0 * cfact(l - m) / cfact(l + m));
     }
+    __host__ __device__ constexpr float operator()(int l, int m) const { return v[l * (LMAX + 1) + m]; }
 };
 
 // Q_l^m(z) at |u| = 1, cos/sin of m phi times sin^m theta, and Yhat.
@@ -65,13 +75,14 @@ struct Harm {
     }
     // Yhat for l >= 1, index l*l-1 + {0: m=0, 2m-1: cos m, 2m: sin m}
     __device__ __forceinline__ void yhat(float* Y) const {
+        constexpr KTab<LMAX> kfac{};
 #pragma unroll
         for (int l = 1; l <= LMAX; ++l) {
             const int o = l * l - 1;
             Y[o] = Q[T::qi(l, 0)];
 #pragma unroll
             for (int m = 1; m <= l; ++m) {
-                const float kq = T::kfac(l, m) * Q[T::qi(l, m)];
+                const float kq = kfac(l, m) * Q[T::qi(l, m)];
                 Y[o + 2 * m - 1] = kq * C[m];
                 Y[o + 2 * m] = kq * S[m];
             }
@@ -88,70 +99,64 @@ struct DescCfg {
     static_assert(NANG <= NRAD, "nang must not exceed nrad");
 };
 
-// Geometry of one pair in the frame centred on particle i.
-struct PairGeom {
-    Mat3 Ri, Rj;
+// One pair in the frame centred on particle i: orientations, t = X_j - X_i,
+// and the centroid of the 2*Ns sites,
+//   c = (R(q_i) sum_k b_k + R(q_j) sum_k b_k + Ns t) / (2 Ns).
+struct PairIn {
+    float4 qi, qj;
     float tx, ty, tz;
-    float cx, cy, cz;  // centroid of the 2*Ns sites
+    float cx, cy, cz;
 };
 
 template <class Cfg>
-__device__ __forceinline__ PairGeom pair_geom(const nepx_pair_model& m, float4 qi, float4 qj,
-                                              float tx, float ty, float tz) {
-    PairGeom g;
-    g.Ri = quat_matrix(qi.x, qi.y, qi.z, qi.w);
-    g.Rj = quat_matrix(qj.x, qj.y, qj.z, qj.w);
-    g.tx = tx;
-    g.ty = ty;
-    g.tz = tz;
-    // centroid = (R_i sum b + R_j sum b + Ns t) / (2 Ns), accumulated site by
-    // site in the same order as the cloud.
+__device__ __forceinline__ PairIn pair_in(const nepx_pair_model& m, float4 qi, float4 qj, float tx, float ty,
+                                          float tz) {
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for (int a = 0; a < Cfg::NS; ++a) {
-        float px, py, pz;
-        mat_vec(g.Ri, m.sites[3 * a], m.sites[3 * a + 1], m.sites[3 * a + 2], px, py, pz);
-        sx += px;
-        sy += py;
-        sz += pz;
+    for (int k = 0; k < Cfg::NS; ++k) {
+        sx += m.sites[3 * k];
+        sy += m.sites[3 * k + 1];
+        sz += m.sites[3 * k + 2];
     }
-    for (int a = 0; a < Cfg::NS; ++a) {
-        float px, py, pz;
-        mat_vec(g.Rj, m.sites[3 * a], m.sites[3 * a + 1], m.sites[3 * a + 2], px, py, pz);
-        sx += px + tx;
-        sy += py + ty;
-        sz += pz + tz;
-    }
+    PairIn p;
+    p.qi = qi;
+    p.qj = qj;
+    p.tx = tx;
+    p.ty = ty;
+    p.tz = tz;
+    float ax, ay, az, bx, by, bz;
+    mat_vec(quat_matrix(qi.x, qi.y, qi.z, qi.w), sx, sy, sz, ax, ay, az);
+    mat_vec(quat_matrix(qj.x, qj.y, qj.z, qj.w), sx, sy, sz, bx, by, bz);
     const float inv = 1.f / (float)Cfg::ND;
-    g.cx = sx * inv;
-    g.cy = sy * inv;
-    g.cz = sz * inv;
-    return g;
+    p.cx = (ax + bx) * inv + 0.5f * tx;
+    p.cy = (ay + by) * inv + 0.5f * ty;
+    p.cz = (az + bz) * inv + 0.5f * tz;
+    return p;
 }
 
-// Site a of the cloud (a < Ns: body i, else body j), relative to the centroid.
-// Also returns the lever arm about the owning body's centre.
-template <class Cfg>
-__device__ __forceinline__ void site_rel(const nepx_pair_model& m, const PairGeom& g, int a,
-                                         float& rx, float& ry, float& rz, float& ax, float& ay,
-                                         float& az) {
-    const int k = a < Cfg::NS ? a : a - Cfg::NS;
-    const float bx = m.sites[3 * k], by = m.sites[3 * k + 1], bz = m.sites[3 * k + 2];
-    if (a < Cfg::NS) {
-        mat_vec(g.Ri, bx, by, bz, ax, ay, az);
-        rx = ax - g.cx;
-        ry = ay - g.cy;
-        rz = az - g.cz;
-    } else {
-        mat_vec(g.Rj, bx, by, bz, ax, ay, az);
-        rx = (ax + g.tx) - g.cx;
-        ry = (ay + g.ty) - g.cy;
-        rz = (az + g.tz) - g.cz;
-    }
+// Site geometry shared by forward and adjoint.
+struct SiteGeo {
+    float ax, ay, az;  // lever arm about the owning body's centre: R b_k
+    float ux, uy, uz;  // unit vector from the centroid
+    float r, ir, rho;  // |r_a|, 1/|r_a|, |r_a|/r_nep
+};
+
+__device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3& R, int k, float ox, float oy,
+                                            float oz, float inv_rn) {
+    SiteGeo s;
+    mat_vec(R, m.sites[3 * k], m.sites[3 * k + 1], m.sites[3 * k + 2], s.ax, s.ay, s.az);
+    const float rx = s.ax + ox, ry = s.ay + oy, rz = s.az + oz;
+    s.r = sqrtf(rx * rx + ry * ry + rz * rz);
+    s.ir = 1.f / s.r;
+    s.ux = rx * s.ir;
+    s.uy = ry * s.ir;
+    s.uz = rz * s.ir;
+    s.rho = s.r * inv_rn;
+    return s;
 }
 
-// Forward: A[n][lm] and g_rad; returns G (D floats).
+// Forward accumulators: g_rad[n] and A[n][lm] = sum_a f_n(a) Yhat_lm(u_a).
 template <class Cfg>
-__device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const PairGeom& g,
+__device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const PairIn& p,
                                                 float (&grad)[Cfg::NRAD + 1],
                                                 float (&A)[Cfg::NANG + 1][Cfg::NLM]) {
 #pragma unroll
@@ -162,35 +167,39 @@ __device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const 
         for (int k = 0; k < Cfg::NLM; ++k) A[n][k] = 0.f;
     const float inv_rn = 1.f / m.r_nep;
 #pragma unroll 1
-    for (int a = 0; a < Cfg::ND; ++a) {
-        float rx, ry, rz, ax, ay, az;
-        site_rel<Cfg>(m, g, a, rx, ry, rz, ax, ay, az);
-        const float r = sqrtf(rx * rx + ry * ry + rz * rz);
-        const float ir = 1.f / r;
-        const float ux = rx * ir, uy = ry * ir, uz = rz * ir;
-        const float rho = r * inv_rn;
-        const float x = 2.f * (rho - 1.f) * (rho - 1.f) - 1.f;
-        const float fc = 0.5f * (1.f + cospif(rho));
-        Harm<Cfg::LMAX> h;
-        h.eval(ux, uy, uz);
-        float Y[Cfg::NLM];
-        h.yhat(Y);
-        float t0 = 1.f, t1 = x;
-#pragma unroll
-        for (int n = 0; n <= Cfg::NRAD; ++n) {
-            float tn;
-            if (n == 0) tn = 1.f;
-            else if (n == 1) tn = x;
-            else {
-                tn = 2.f * x * t1 - t0;
-                t0 = t1;
-                t1 = tn;
+    for (int body = 0; body < 2; ++body) {
+        const float4 q = body ? p.qj : p.qi;
+        const Mat3 R = quat_matrix(q.x, q.y, q.z, q.w);
+        const float ox = (body ? p.tx : 0.f) - p.cx, oy = (body ? p.ty : 0.f) - p.cy,
+                    oz = (body ? p.tz : 0.f) - p.cz;
+#pragma unroll 1
+        for (int k = 0; k < Cfg::NS; ++k) {
+            const SiteGeo s = site_geo(m, R, k, ox, oy, oz, inv_rn);
+            const float x = 2.f * (s.rho - 1.f) * (s.rho - 1.f) - 1.f;
+            const float fc = 0.5f * (1.f + cospif(s.rho));
+            float Y[Cfg::NLM];
+            {
+                Harm<Cfg::LMAX> h;
+                h.eval(s.ux, s.uy, s.uz);
+                h.yhat(Y);
             }
-            const float f = 0.5f * (tn + 1.f) * fc;
-            grad[n] += f;
-            if (n <= Cfg::NANG) {
+            float t0 = 1.f, t1 = x;
 #pragma unroll
-                for (int k = 0; k < Cfg::NLM; ++k) A[n][k] += f * Y[k];
+            for (int n = 0; n <= Cfg::NRAD; ++n) {
+                float tn;
+                if (n == 0) tn = 1.f;
+                else if (n == 1) tn = x;
+                else {
+                    tn = 2.f * x * t1 - t0;
+                    t0 = t1;
+                    t1 = tn;
+                }
+                const float f = 0.5f * (tn + 1.f) * fc;
+                grad[n] += f;
+                if (n <= Cfg::NANG) {
+#pragma unroll
+                    for (int kk = 0; kk < Cfg::NLM; ++kk) A[n][kk] += f * Y[kk];
+                }
             }
         }
     }
@@ -221,13 +230,19 @@ struct PairOut {
     float tjx, tjy, tjz;  // torque on j about X_j
 };
 
-// Adjoint: given dE/dG (already scaled to energy units), per-pair force and
-// torques.  A is the forward accumulator (overwritten by dE/dA).
+// Adjoint.  dG = dE/dG = [dE/dg_rad | dE/dg_ang] (already in energy units).  A is
+// the forward accumulator, overwritten by B = dE/dA = 2 dE/dg_ang[n,l] A.
+// Per site a (r_a relative to the centroid, u_a = r_a/|r_a|):
+//   dE/df_n(a) = dgr[n] + sum_lm B_nlm Yhat_lm(u_a)            (n <= nang)
+//   c_lm(a)    = sum_n B_nlm f_n(a)
+//   dE/dr_a    = (sum_n dE/df_n f_n'(rho)) u_a / r_nep
+//              + (1/|r_a|) [ sum_lm c_lm grad S_lm(u_a) - u_a sum_l l sum_m c_lm Yhat_lm(u_a) ]
+// (S_lm: the homogeneous polynomial with S_lm(u) = Yhat_lm(u) on |u| = 1), and
+// dE/dp_a = dE/dr_a - mean_b dE/dr_b through the centroid.
 template <class Cfg>
-__device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const PairGeom& g,
+__device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const PairIn& p,
                                                 const float (&dG)[Cfg::D],
                                                 float (&A)[Cfg::NANG + 1][Cfg::NLM]) {
-    // dE/dA_nlm = 2 dE/dg_ang[n,l] A_nlm
 #pragma unroll
     for (int n = 0; n <= Cfg::NANG; ++n) {
 #pragma unroll
@@ -238,121 +253,132 @@ __device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const 
         }
     }
     using HT = HarmTables<Cfg::LMAX>;
+    constexpr KTab<Cfg::LMAX> kfac{};
     const float inv_rn = 1.f / m.r_nep;
-    float Gix = 0.f, Giy = 0.f, Giz = 0.f, Gjx = 0.f, Gjy = 0.f, Gjz = 0.f;
-    float Tix = 0.f, Tiy = 0.f, Tiz = 0.f, Tjx = 0.f, Tjy = 0.f, Tjz = 0.f;
-    float Aix = 0.f, Aiy = 0.f, Aiz = 0.f, Ajx = 0.f, Ajy = 0.f, Ajz = 0.f;
+    float Gb[2][3], Tb[2][3], Ab[2][3];
 #pragma unroll 1
-    for (int a = 0; a < Cfg::ND; ++a) {
-        float rx, ry, rz, ax, ay, az;
-        site_rel<Cfg>(m, g, a, rx, ry, rz, ax, ay, az);
-        const float r = sqrtf(rx * rx + ry * ry + rz * rz);
-        const float ir = 1.f / r;
-        const float ux = rx * ir, uy = ry * ir, uz = rz * ir;
-        const float rho = r * inv_rn;
-        const float xm = rho - 1.f;
-        const float x = 2.f * xm * xm - 1.f;
-        const float dx = 4.f * xm;
-        float sp, cp;
-        sincospif(rho, &sp, &cp);
-        const float fc = 0.5f * (1.f + cp);
-        const float dfc = -0.5f * (float)M_PI * sp;
-        Harm<Cfg::LMAX> h;
-        h.eval(ux, uy, uz);
-        float Y[Cfg::NLM];
-        h.yhat(Y);
-        float c[Cfg::NLM];
+    for (int body = 0; body < 2; ++body) {
+        const float4 q = body ? p.qj : p.qi;
+        const Mat3 R = quat_matrix(q.x, q.y, q.z, q.w);
+        const float ox = (body ? p.tx : 0.f) - p.cx, oy = (body ? p.ty : 0.f) - p.cy,
+                    oz = (body ? p.tz : 0.f) - p.cz;
+        float gx_s = 0.f, gy_s = 0.f, gz_s = 0.f, tx_s = 0.f, ty_s = 0.f, tz_s = 0.f;
+        float ax_s = 0.f, ay_s = 0.f, az_s = 0.f;
+#pragma unroll 1
+        for (int k = 0; k < Cfg::NS; ++k) {
+            const SiteGeo s = site_geo(m, R, k, ox, oy, oz, inv_rn);
+            const float xm = s.rho - 1.f;
+            const float x = 2.f * xm * xm - 1.f;
+            const float dx = 4.f * xm;
+            float sp, cp;
+            sincospif(s.rho, &sp, &cp);
+            const float fc = 0.5f * (1.f + cp);
+            const float dfc = -0.5f * (float)M_PI * sp;
+            float c[Cfg::NLM];
+            float dEdrho = 0.f, proj = 0.f;
+            {
+                float Y[Cfg::NLM];
+                {
+                    Harm<Cfg::LMAX> h;
+                    h.eval(s.ux, s.uy, s.uz);
+                    h.yhat(Y);
+                }
 #pragma unroll
-        for (int k = 0; k < Cfg::NLM; ++k) c[k] = 0.f;
-        float dEdrho = 0.f;
-        float t0 = 1.f, t1 = x, d0 = 0.f, d1 = 1.f;
+                for (int kk = 0; kk < Cfg::NLM; ++kk) c[kk] = 0.f;
+                float t0 = 1.f, t1 = x, d0 = 0.f, d1 = 1.f;
 #pragma unroll
-        for (int n = 0; n <= Cfg::NRAD; ++n) {
-            float tn, dn;
-            if (n == 0) {
-                tn = 1.f;
-                dn = 0.f;
-            } else if (n == 1) {
-                tn = x;
-                dn = 1.f;
-            } else {
-                tn = 2.f * x * t1 - t0;
-                dn = 2.f * t1 + 2.f * x * d1 - d0;
-                t0 = t1;
-                t1 = tn;
-                d0 = d1;
-                d1 = dn;
+                for (int n = 0; n <= Cfg::NRAD; ++n) {
+                    float tn, dn;
+                    if (n == 0) {
+                        tn = 1.f;
+                        dn = 0.f;
+                    } else if (n == 1) {
+                        tn = x;
+                        dn = 1.f;
+                    } else {
+                        tn = 2.f * x * t1 - t0;
+                        dn = 2.f * t1 + 2.f * x * d1 - d0;
+                        t0 = t1;
+                        t1 = tn;
+                        d0 = d1;
+                        d1 = dn;
+                    }
+                    const float f = 0.5f * (tn + 1.f) * fc;
+                    const float df = 0.5f * dn * dx * fc + 0.5f * (tn + 1.f) * dfc;
+                    float sn = dG[n];
+                    if (n <= Cfg::NANG) {
+#pragma unroll
+                        for (int kk = 0; kk < Cfg::NLM; ++kk) {
+                            sn += A[n][kk] * Y[kk];
+                            c[kk] += A[n][kk] * f;
+                        }
+                    }
+                    dEdrho += sn * df;
+                }
+#pragma unroll
+                for (int l = 1; l <= Cfg::LMAX; ++l)
+#pragma unroll
+                    for (int kk = l * l - 1; kk < (l + 1) * (l + 1) - 1; ++kk) proj += (float)l * c[kk] * Y[kk];
             }
-            const float f = 0.5f * (tn + 1.f) * fc;
-            const float df = 0.5f * dn * dx * fc + 0.5f * (tn + 1.f) * dfc;
-            float s = dG[n];
-            if (n <= Cfg::NANG) {
+            // gradient of sum_lm c_lm S_lm at u (harmonic recurrences recomputed)
+            Harm<Cfg::LMAX> h;
+            h.eval(s.ux, s.uy, s.uz);
+            float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
-                for (int k = 0; k < Cfg::NLM; ++k) {
-                    s += A[n][k] * Y[k];
-                    c[k] += A[n][k] * f;
+            for (int l = 1; l <= Cfg::LMAX; ++l) {
+                const int o = l * l - 1;
+                {  // m = 0
+                    const float cc = c[o];
+                    const float qa = (l - 1 >= 1) ? h.Q[HT::qi(l - 1, 1)] : 0.f;
+                    gx += cc * s.ux * qa;
+                    gy += cc * s.uy * qa;
+                    gz += cc * (float)l * h.Q[HT::qi(l - 1, 0)];
+                }
+#pragma unroll
+                for (int mm = 1; mm <= l; ++mm) {
+                    const float K = kfac(l, mm);
+                    const float cc = c[o + 2 * mm - 1] * K, cs = c[o + 2 * mm] * K;
+                    const float qa = (mm + 1 <= l - 1) ? h.Q[HT::qi(l - 1, mm + 1)] : 0.f;
+                    const float qb = h.Q[HT::qi(l, mm)];
+                    const float qc = (mm <= l - 1) ? h.Q[HT::qi(l - 1, mm)] : 0.f;
+                    const float e = cc * h.C[mm] + cs * h.S[mm];
+                    gx += s.ux * qa * e + (float)mm * qb * (cc * h.C[mm - 1] + cs * h.S[mm - 1]);
+                    gy += s.uy * qa * e + (float)mm * qb * (cs * h.C[mm - 1] - cc * h.S[mm - 1]);
+                    gz += (float)(l + mm) * qc * e;
                 }
             }
-            dEdrho += s * df;
+            const float rad = dEdrho * inv_rn;
+            const float ga_x = rad * s.ux + s.ir * (gx - proj * s.ux);
+            const float ga_y = rad * s.uy + s.ir * (gy - proj * s.uy);
+            const float ga_z = rad * s.uz + s.ir * (gz - proj * s.uz);
+            gx_s += ga_x;
+            gy_s += ga_y;
+            gz_s += ga_z;
+            tx_s += s.ay * ga_z - s.az * ga_y;
+            ty_s += s.az * ga_x - s.ax * ga_z;
+            tz_s += s.ax * ga_y - s.ay * ga_x;
+            ax_s += s.ax;
+            ay_s += s.ay;
+            az_s += s.az;
         }
-        // gradient of sum_lm c_lm Yhat_lm(r/|r|) with respect to r
-        float gx = 0.f, gy = 0.f, gz = 0.f, proj = 0.f;
-#pragma unroll
-        for (int l = 1; l <= Cfg::LMAX; ++l) {
-            const int o = l * l - 1;
-            {  // m = 0
-                const float cc = c[o];
-                const float qa = (l - 1 >= 1) ? h.Q[HT::qi(l - 1, 1)] : 0.f;
-                gx += cc * ux * qa;
-                gy += cc * uy * qa;
-                gz += cc * (float)l * h.Q[HT::qi(l - 1, 0)];
-                proj += (float)l * cc * Y[o];
-            }
-#pragma unroll
-            for (int mm = 1; mm <= l; ++mm) {
-                const float K = HT::kfac(l, mm);
-                const float cc = c[o + 2 * mm - 1] * K, cs = c[o + 2 * mm] * K;
-                const float qa = (mm + 1 <= l - 1) ? h.Q[HT::qi(l - 1, mm + 1)] : 0.f;
-                const float qb = h.Q[HT::qi(l, mm)];
-                const float qc = (mm <= l - 1) ? h.Q[HT::qi(l - 1, mm)] : 0.f;
-                const float e = cc * h.C[mm] + cs * h.S[mm];
-                gx += ux * qa * e + (float)mm * qb * (cc * h.C[mm - 1] + cs * h.S[mm - 1]);
-                gy += uy * qa * e + (float)mm * qb * (cs * h.C[mm - 1] - cc * h.S[mm - 1]);
-                gz += (float)(l + mm) * qc * e;
-                proj += (float)l * (c[o + 2 * mm - 1] * Y[o + 2 * mm - 1] + c[o + 2 * mm] * Y[o + 2 * mm]);
-            }
-        }
-        const float rad = dEdrho * inv_rn;
-        const float ga_x = rad * ux + ir * (gx - proj * ux);
-        const float ga_y = rad * uy + ir * (gy - proj * uy);
-        const float ga_z = rad * uz + ir * (gz - proj * uz);
-        if (a < Cfg::NS) {
-            Gix += ga_x; Giy += ga_y; Giz += ga_z;
-            Tix += ay * ga_z - az * ga_y;
-            Tiy += az * ga_x - ax * ga_z;
-            Tiz += ax * ga_y - ay * ga_x;
-            Aix += ax; Aiy += ay; Aiz += az;
-        } else {
-            Gjx += ga_x; Gjy += ga_y; Gjz += ga_z;
-            Tjx += ay * ga_z - az * ga_y;
-            Tjy += az * ga_x - ax * ga_z;
-            Tjz += ax * ga_y - ay * ga_x;
-            Ajx += ax; Ajy += ay; Ajz += az;
-        }
+        Gb[body][0] = gx_s; Gb[body][1] = gy_s; Gb[body][2] = gz_s;
+        Tb[body][0] = tx_s; Tb[body][1] = ty_s; Tb[body][2] = tz_s;
+        Ab[body][0] = ax_s; Ab[body][1] = ay_s; Ab[body][2] = az_s;
     }
     // dE/dp_a = g_a - gbar ; F_i = -sum_{a in i} dE/dp_a ; tau = -sum arm x dE/dp
     const float inv_nd = 1.f / (float)Cfg::ND;
-    const float bx = (Gix + Gjx) * inv_nd, by = (Giy + Gjy) * inv_nd, bz = (Giz + Gjz) * inv_nd;
+    const float bx = (Gb[0][0] + Gb[1][0]) * inv_nd, by = (Gb[0][1] + Gb[1][1]) * inv_nd,
+                bz = (Gb[0][2] + Gb[1][2]) * inv_nd;
     PairOut o;
-    o.fx = -(Gix - (float)Cfg::NS * bx);
-    o.fy = -(Giy - (float)Cfg::NS * by);
-    o.fz = -(Giz - (float)Cfg::NS * bz);
-    o.tix = -(Tix - (Aiy * bz - Aiz * by));
-    o.tiy = -(Tiy - (Aiz * bx - Aix * bz));
-    o.tiz = -(Tiz - (Aix * by - Aiy * bx));
-    o.tjx = -(Tjx - (Ajy * bz - Ajz * by));
-    o.tjy = -(Tjy - (Ajz * bx - Ajx * bz));
-    o.tjz = -(Tjz - (Ajx * by - Ajy * bx));
+    o.fx = -(Gb[0][0] - (float)Cfg::NS * bx);
+    o.fy = -(Gb[0][1] - (float)Cfg::NS * by);
+    o.fz = -(Gb[0][2] - (float)Cfg::NS * bz);
+    o.tix = -(Tb[0][0] - (Ab[0][1] * bz - Ab[0][2] * by));
+    o.tiy = -(Tb[0][1] - (Ab[0][2] * bx - Ab[0][0] * bz));
+    o.tiz = -(Tb[0][2] - (Ab[0][0] * by - Ab[0][1] * bx));
+    o.tjx = -(Tb[1][0] - (Ab[1][1] * bz - Ab[1][2] * by));
+    o.tjy = -(Tb[1][1] - (Ab[1][2] * bx - Ab[1][0] * bz));
+    o.tjz = -(Tb[1][2] - (Ab[1][0] * by - Ab[1][1] * bx));
     return o;
 }
 

@@ -66,6 +66,9 @@ __device__ __forceinline__ float ffnn_tile(const float* __restrict__ W, int act,
                                            const v4f (&gin)[N::MD], float scale,
                                            v4f (&dgo)[N::MD]) {
     constexpr NetLayout Lay = N::layout();
+    // Keep the weight reads inside the tile: without this the compiler hoists
+    // every loop-invariant A-operand load out of the caller's loops and spills.
+    asm volatile("" : "+s"(W));
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4;
     v4f h[N::NL][N::MT];  // activations of every hidden layer (D layout)

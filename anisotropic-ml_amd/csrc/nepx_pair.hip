@@ -9,6 +9,12 @@ namespace nepx {
 
 constexpr int kBlock = 256;
 
+// Occupancy target of the register-heavy adjoint/fused kernels (waves per SIMD).
+#ifndef NEPX_ADJ_WAVES
+#define NEPX_ADJ_WAVES 1
+#endif
+#define NEPX_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(NEPX_ADJ_WAVES, NEPX_ADJ_WAVES)))
+
 __device__ __forceinline__ int n_pairs(const int32_t* np_dev, int64_t maxp) {
     const int p = *np_dev;
     return p < maxp ? p : (int)maxp;
@@ -74,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_desc_fwd(nepx_pair_model m, const fl
     for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < P; idx += gridDim.x * kBlock) {
         float4 qi, qj, t;
         load_pair(qf, pi, pj, pt, idx, qi, qj, t);
-        const PairGeom g = pair_geom<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
         float grad[Cfg::NRAD + 1];
         float A[Cfg::NANG + 1][Cfg::NLM];
         desc_accumulate<Cfg>(m, g, grad, A);
@@ -85,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_desc_fwd(nepx_pair_model m, const fl
 }
 
 template <class Cfg>
-__global__ __launch_bounds__(kBlock) void k_adjoint(nepx_pair_model m, const float4* __restrict__ qf,
+__global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_model m, const float4* __restrict__ qf,
                                                     const int32_t* __restrict__ pi,
                                                     const int32_t* __restrict__ pj,
                                                     const float4* __restrict__ pt,
@@ -97,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(nepx_pair_model m, const flo
     for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < P; idx += gridDim.x * kBlock) {
         float4 qi, qj, t;
         load_pair(qf, pi, pj, pt, idx, qi, qj, t);
-        const PairGeom g = pair_geom<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
         float grad[Cfg::NRAD + 1];
         float A[Cfg::NANG + 1][Cfg::NLM];
         desc_accumulate<Cfg>(m, g, grad, A);
@@ -163,8 +169,11 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ W, in
 // Fused: descriptor -> LDS -> MFMA network -> LDS -> adjoint.  Each block
 // takes 256 consecutive pairs per iteration (64 per wave), so every wave of a
 // block runs the same number of iterations and the barriers are uniform.
+// Only the pair index survives from one phase to the next: the adjoint phase
+// reloads the pair and recomputes its forward accumulators, which keeps the
+// register footprint of the kernel at that of its largest phase.
 template <class Cfg, class N>
-__global__ __launch_bounds__(kBlock) void k_fused(nepx_pair_model m, const float* __restrict__ W,
+__global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model m, const float* __restrict__ W,
                                                   int act, float en_min, float scale,
                                                   const float4* __restrict__ qf,
                                                   const int32_t* __restrict__ pi,
@@ -182,10 +191,10 @@ __global__ __launch_bounds__(kBlock) void k_fused(nepx_pair_model m, const float
     for (int base = blockIdx.x * kBlock; base < P; base += gridDim.x * kBlock) {
         const int idx = base + threadIdx.x;
         const bool valid = idx < P;
-        float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
-        if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
-        const PairGeom g = pair_geom<Cfg>(m, qi, qj, t.x, t.y, t.z);
         {
+            float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
+            if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+            const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
             float grad[Cfg::NRAD + 1];
             float A[Cfg::NANG + 1][Cfg::NLM];
             desc_accumulate<Cfg>(m, g, grad, A);
@@ -208,6 +217,9 @@ __global__ __launch_bounds__(kBlock) void k_fused(nepx_pair_model m, const float
         }
         __syncthreads();
         if (valid) {
+            float4 qi, qj, t;
+            load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+            const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
             float dG[Cfg::D];
 #pragma unroll
             for (int k = 0; k < Cfg::D; ++k) dG[k] = myrow[k];
