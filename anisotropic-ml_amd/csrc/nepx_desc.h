@@ -255,7 +255,9 @@ __device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const 
     using HT = HarmTables<Cfg::LMAX>;
     constexpr KTab<Cfg::LMAX> kfac{};
     const float inv_rn = 1.f / m.r_nep;
-    float Gb[2][3], Tb[2][3], Ab[2][3];
+    // per-body sums of dE/dr_a, arm x dE/dr_a and arms (body 0 = i, 1 = j)
+    float G0x = 0.f, G0y = 0.f, G0z = 0.f, T0x = 0.f, T0y = 0.f, T0z = 0.f, A0x = 0.f, A0y = 0.f, A0z = 0.f;
+    float G1x = 0.f, G1y = 0.f, G1z = 0.f, T1x = 0.f, T1y = 0.f, T1z = 0.f, A1x = 0.f, A1y = 0.f, A1z = 0.f;
 #pragma unroll 1
     for (int body = 0; body < 2; ++body) {
         const float4 q = body ? p.qj : p.qi;
@@ -361,24 +363,25 @@ __device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const 
             ay_s += s.ay;
             az_s += s.az;
         }
-        Gb[body][0] = gx_s; Gb[body][1] = gy_s; Gb[body][2] = gz_s;
-        Tb[body][0] = tx_s; Tb[body][1] = ty_s; Tb[body][2] = tz_s;
-        Ab[body][0] = ax_s; Ab[body][1] = ay_s; Ab[body][2] = az_s;
+        if (body == 0) {
+            G0x = gx_s; G0y = gy_s; G0z = gz_s; T0x = tx_s; T0y = ty_s; T0z = tz_s; A0x = ax_s; A0y = ay_s; A0z = az_s;
+        } else {
+            G1x = gx_s; G1y = gy_s; G1z = gz_s; T1x = tx_s; T1y = ty_s; T1z = tz_s; A1x = ax_s; A1y = ay_s; A1z = az_s;
+        }
     }
     // dE/dp_a = g_a - gbar ; F_i = -sum_{a in i} dE/dp_a ; tau = -sum arm x dE/dp
     const float inv_nd = 1.f / (float)Cfg::ND;
-    const float bx = (Gb[0][0] + Gb[1][0]) * inv_nd, by = (Gb[0][1] + Gb[1][1]) * inv_nd,
-                bz = (Gb[0][2] + Gb[1][2]) * inv_nd;
+    const float bx = (G0x + G1x) * inv_nd, by = (G0y + G1y) * inv_nd, bz = (G0z + G1z) * inv_nd;
     PairOut o;
-    o.fx = -(Gb[0][0] - (float)Cfg::NS * bx);
-    o.fy = -(Gb[0][1] - (float)Cfg::NS * by);
-    o.fz = -(Gb[0][2] - (float)Cfg::NS * bz);
-    o.tix = -(Tb[0][0] - (Ab[0][1] * bz - Ab[0][2] * by));
-    o.tiy = -(Tb[0][1] - (Ab[0][2] * bx - Ab[0][0] * bz));
-    o.tiz = -(Tb[0][2] - (Ab[0][0] * by - Ab[0][1] * bx));
-    o.tjx = -(Tb[1][0] - (Ab[1][1] * bz - Ab[1][2] * by));
-    o.tjy = -(Tb[1][1] - (Ab[1][2] * bx - Ab[1][0] * bz));
-    o.tjz = -(Tb[1][2] - (Ab[1][0] * by - Ab[1][1] * bx));
+    o.fx = -(G0x - (float)Cfg::NS * bx);
+    o.fy = -(G0y - (float)Cfg::NS * by);
+    o.fz = -(G0z - (float)Cfg::NS * bz);
+    o.tix = -(T0x - (A0y * bz - A0z * by));
+    o.tiy = -(T0y - (A0z * bx - A0x * bz));
+    o.tiz = -(T0z - (A0x * by - A0y * bx));
+    o.tjx = -(T1x - (A1y * bz - A1z * by));
+    o.tjy = -(T1y - (A1z * bx - A1x * bz));
+    o.tjz = -(T1z - (A1x * by - A1y * bx));
     return o;
 }
 

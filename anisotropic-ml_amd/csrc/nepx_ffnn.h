@@ -57,101 +57,132 @@ __device__ __forceinline__ float act_der(float h, int act) {  // in terms of the
     return act == NEPX_ACT_TANH ? 1.f - h * h : (h > 0.f ? 1.f : 0.f);
 }
 
-// One tile of 16 pairs.  gin: lane's 4-float chunks of its pair's input
-// (DP/16 chunks, chunk c holds inputs 16c + 4*(lane/16) + 0..3).
-// Outputs: y (network output, valid in every lane for pair lane%16) and
-// dgo (dE/dG chunks in the same layout), scaled by `scale`.
-template <class N>
-__device__ __forceinline__ float ffnn_tile(const float* __restrict__ W, int act,
-                                           const v4f (&gin)[N::MD], float scale,
-                                           v4f (&dgo)[N::MD]) {
+// NT tiles of 16 pairs each, sharing every A-operand read.  gin[t]: lane's
+// 4-float chunks of its pair's input in tile t (chunk c holds inputs
+// 16c + 4*(lane/16) + 0..3).  Outputs: y[t] (network output, valid in every
+// lane for pair lane%16 of tile t) and dgo[t] (dE/dG chunks in the same
+// layout), both scaled by `scale`.  W may point to global memory or to a copy
+// in LDS.
+template <class N, int NT>
+__device__ __forceinline__ void ffnn_tiles(const float* __restrict__ W, int act, const v4f (&gin)[NT][N::MD],
+                                           float scale, v4f (&dgo)[NT][N::MD], float (&y)[NT]) {
     constexpr NetLayout Lay = N::layout();
-    // Keep the weight reads inside the tile: without this the compiler hoists
-    // every loop-invariant A-operand load out of the caller's loops and spills.
-    asm volatile("" : "+s"(W));
-    const int lane = threadIdx.x & 63;
+    // An opaque lane index per call keeps the (loop-invariant) weight reads
+    // inside the caller's loop; hoisted, they would occupy hundreds of registers.
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
     const int g = lane >> 4;
-    v4f h[N::NL][N::MT];  // activations of every hidden layer (D layout)
+    v4f h[N::NL][NT][N::MT];  // activations of every hidden layer (D layout)
     // ---- forward ----
 #pragma unroll
     for (int k = 0; k < N::NL; ++k) {
         const float* Wf = W + Lay.fwd_off(k);
         const float* bk = W + Lay.bias_off(k);
         const int ks = (k == 0) ? N::KD : N::KH;
-        v4f acc[N::MT];
+        v4f acc[NT][N::MT];
 #pragma unroll
         for (int mt = 0; mt < N::MT; ++mt) {
             const float4 b4 = *reinterpret_cast<const float4*>(bk + 16 * mt + 4 * g);
-            acc[mt] = v4f{b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][mt] = v4f{b4.x, b4.y, b4.z, b4.w};
         }
 #pragma unroll
         for (int s = 0; s < ks; ++s) {
-            const float bop = (k == 0) ? gin[s >> 2][s & 3] : h[k - 1][s >> 2][s & 3];
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) {
                 const float aop = Wf[(mt * ks + s) * 64 + lane];
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, acc[mt], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const float bop = (k == 0) ? gin[t][s >> 2][s & 3] : h[k - 1][t][s >> 2][s & 3];
+                    acc[t][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, acc[t][mt], 0, 0, 0);
+                }
             }
         }
 #pragma unroll
-        for (int mt = 0; mt < N::MT; ++mt)
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) h[k][mt][r] = act_fwd(acc[mt][r], act);
+            for (int mt = 0; mt < N::MT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) h[k][t][mt][r] = act_fwd(acc[t][mt][r], act);
     }
     // ---- output layer (dot over units, reduced across the 4 lane groups) ----
     const float* wo = W + Lay.wout_off();
-    float y = 0.f;
-    v4f delta[N::MT];
+    v4f delta[NT][N::MT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) y[t] = 0.f;
 #pragma unroll
     for (int mt = 0; mt < N::MT; ++mt) {
         const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * mt + 4 * g);
         const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            y += wv[r] * h[N::NL - 1][mt][r];
-            delta[mt][r] = scale * wv[r] * act_der(h[N::NL - 1][mt][r], act);
-        }
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                y[t] += wv[r] * h[N::NL - 1][t][mt][r];
+                delta[t][mt][r] = scale * wv[r] * act_der(h[N::NL - 1][t][mt][r], act);
+            }
     }
-    y += __shfl_xor(y, 16);
-    y += __shfl_xor(y, 32);
-    y += W[Lay.bout_off()];
+    const float bo = W[Lay.bout_off()];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        y[t] += __shfl_xor(y[t], 16);
+        y[t] += __shfl_xor(y[t], 32);
+        y[t] += bo;
+    }
     // ---- backward through the hidden layers ----
 #pragma unroll
     for (int k = N::NL - 1; k >= 1; --k) {
         const float* Wb = W + Lay.bwd_off(k);
-        v4f acc[N::MT];
+        v4f acc[NT][N::MT];
 #pragma unroll
-        for (int mt = 0; mt < N::MT; ++mt) acc[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt) acc[t][mt] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < N::KH; ++s) {
-            const float bop = delta[s >> 2][s & 3];
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) {
                 const float aop = Wb[(mt * N::KH + s) * 64 + lane];
-                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, acc[mt], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[t][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, delta[t][s >> 2][s & 3], acc[t][mt], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int mt = 0; mt < N::MT; ++mt)
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) delta[mt][r] = acc[mt][r] * act_der(h[k - 1][mt][r], act);
+            for (int mt = 0; mt < N::MT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) delta[t][mt][r] = acc[t][mt][r] * act_der(h[k - 1][t][mt][r], act);
     }
     // ---- dE/dG = W_1^T delta_1 ----
     {
         const float* Wb = W + Lay.bwd_off(0);
 #pragma unroll
-        for (int mt = 0; mt < N::MD; ++mt) dgo[mt] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < N::MD; ++mt) dgo[t][mt] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < N::KH; ++s) {
-            const float bop = delta[s >> 2][s & 3];
 #pragma unroll
             for (int mt = 0; mt < N::MD; ++mt) {
                 const float aop = Wb[(mt * N::KH + s) * 64 + lane];
-                dgo[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, bop, dgo[mt], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    dgo[t][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(aop, delta[t][s >> 2][s & 3], dgo[t][mt], 0, 0, 0);
             }
         }
     }
-    return y;
+}
+
+// Copy the packed network into LDS (all threads of the block cooperate).
+template <class N>
+__device__ __forceinline__ void stage_weights(float* __restrict__ lds, const float* __restrict__ W) {
+    constexpr int total = N::layout().total();
+    const float4* src = reinterpret_cast<const float4*>(W);
+    float4* dst = reinterpret_cast<float4*>(lds);
+    for (int k = threadIdx.x; k < total / 4; k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
 }
 
 }  // namespace nepx

@@ -80,7 +80,12 @@ __device__ __forceinline__ double disp2(const nepx_state& st, int i, double L) {
     return dx * dx + dy * dy + dz * dz;
 }
 
-// mode 0: full step; mode 1: kinetic energies only (initialisation).
+// Parts of the step (mode): 0 = whole step; 1 = kinetic energies only
+// (initialisation); 2 = step two (half kicks, torque kick stored in dp);
+// 3 = translation of step one; 4 = rotation of step one (uses dp);
+// 5 = kinetic energies + displacement for the thermostat tail.
+enum { M_STEP = 0, M_KE = 1, M_TWO = 2, M_TRANS = 3, M_ROT = 4, M_THERMO = 5 };
+
 __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
     const int i = blockIdx.x * kIB + threadIdx.x;
     double kt = 0.0, kr = 0.0, u = 0.0, d2 = 0.0;
@@ -90,12 +95,13 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
         const nepx_thermo& th = st.th[sidx];
         const double dt = sp.dt, L = sp.L;
         const double m = st.mass[i];
+        const double theta = th.theta, er = th.e_r;
         double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
         Q4 q{st.q0[i], st.q1[i], st.q2[i], st.q3[i]};
         Q4 p{st.p0[i], st.p1[i], st.p2[i], st.p3[i]};
-        if (mode == 0) {
-            const double theta = th.theta, er = th.e_r;
-            const double ax = st.fx[i] / m, ay = st.fy[i] / m, az = st.fz[i] / m;
+        const double ax = st.fx[i] / m, ay = st.fy[i] / m, az = st.fz[i] / m;
+        Q4 dp{0.0, 0.0, 0.0, 0.0};
+        if (mode == M_STEP || mode == M_TWO) {
             // step two: translation and torque half kicks
             vx = vx * theta + 0.5 * dt * ax;
             vy = vy * theta + 0.5 * dt * ay;
@@ -103,9 +109,23 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
             const Q4 qc{q.w, -q.x, -q.y, -q.z};
             const Q4 tb = qmul(qmul(qc, Q4{0.0, st.tx[i], st.ty[i], st.tz[i]}), q);  // R(q*) tau
             const Q4 dq = qmul(q, Q4{0.0, tb.x, tb.y, tb.z});
-            const Q4 dp{dt * dq.w, dt * dq.x, dt * dq.y, dt * dq.z};
+            dp = Q4{dt * dq.w, dt * dq.x, dt * dq.y, dt * dq.z};
             p = Q4{p.w * er + dp.w, p.x * er + dp.x, p.y * er + dp.y, p.z * er + dp.z};
-            // step one: translation
+            if (mode == M_TWO) {
+                st.vx[i] = vx;
+                st.vy[i] = vy;
+                st.vz[i] = vz;
+                st.p0[i] = p.w;
+                st.p1[i] = p.x;
+                st.p2[i] = p.y;
+                st.p3[i] = p.z;
+                st.dp[4 * (size_t)i] = dp.w;
+                st.dp[4 * (size_t)i + 1] = dp.x;
+                st.dp[4 * (size_t)i + 2] = dp.y;
+                st.dp[4 * (size_t)i + 3] = dp.z;
+            }
+        }
+        if (mode == M_STEP || mode == M_TRANS) {
             vx = (vx + 0.5 * dt * ax) * theta;
             vy = (vy + 0.5 * dt * ay) * theta;
             vz = (vz + 0.5 * dt * az) * theta;
@@ -122,7 +142,11 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
             st.vx[i] = vx;
             st.vy[i] = vy;
             st.vz[i] = vz;
-            // step one: rotation
+        }
+        if (mode == M_STEP || mode == M_ROT) {
+            if (mode == M_ROT)
+                dp = Q4{st.dp[4 * (size_t)i], st.dp[4 * (size_t)i + 1], st.dp[4 * (size_t)i + 2],
+                        st.dp[4 * (size_t)i + 3]};
             p = Q4{(p.w + dp.w) * er, (p.x + dp.x) * er, (p.y + dp.y) * er, (p.z + dp.z) * er};
             rotor(3, 0.5 * dt, sp.moi[2], p, q);
             rotor(2, 0.5 * dt, sp.moi[1], p, q);
@@ -139,14 +163,18 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
             st.p1[i] = p.x;
             st.p2[i] = p.y;
             st.p3[i] = p.z;
-            d2 = disp2(st, i, L);
         }
-        reinterpret_cast<float4*>(st.qf)[i] = make_float4((float)q.w, (float)q.x, (float)q.y, (float)q.z);
-        kt = 0.5 * m * (vx * vx + vy * vy + vz * vz);
-        kr = rot_ke(q, p, sp.moi);
-        u = st.upot ? st.upot[i] : 0.0;
+        if (mode == M_STEP || mode == M_ROT || mode == M_KE)
+            reinterpret_cast<float4*>(st.qf)[i] = make_float4((float)q.w, (float)q.x, (float)q.y, (float)q.z);
+        if (mode == M_STEP || mode == M_THERMO || mode == M_KE) {
+            kt = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+            kr = rot_ke(q, p, sp.moi);
+            u = st.upot ? st.upot[i] : 0.0;
+            if (mode != M_KE) d2 = disp2(st, i, L);
+        }
     }
-    block_reduce4(kt, kr, u, d2, st.partials + 4 * (size_t)blockIdx.x);
+    if (mode == M_STEP || mode == M_THERMO || mode == M_KE)  // uniform per launch
+        block_reduce4(kt, kr, u, d2, st.partials + 4 * (size_t)blockIdx.x);
 }
 
 // One block per system: fixed-order sums of the block partials, thermostat
@@ -187,6 +215,9 @@ __global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* f
             th.eta_r += xr_p * dt;
             th.xi_r = xr;
             th.e_r = exp(-0.5 * dt * xr);
+        } else {  // restored state: the factors follow from the saved xi, xi_r
+            th.theta = exp(-0.5 * th.xi * sp.dt);
+            th.e_r = exp(-0.5 * sp.dt * th.xi_r);
         }
         const double half = 0.5 * sp.skin;
         if (flags && out[3] > half * half) flags[0] = 1;
@@ -216,15 +247,30 @@ extern "C" int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream) {
     return NEPX_OK;
 }
 
-extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, void* stream) {
+extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_thermostat, void* stream) {
     int rc = check_nvt(st);
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
     hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1);
-    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr, 1);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr,
+                       update_thermostat ? 1 : 0);
     NEPX_LAUNCH_CHECK();
     (void)nl;
+    return NEPX_OK;
+}
+
+extern "C" int nepx_nvt_part(nepx_state* st, nepx_nlist* nl, int32_t part, void* stream) {
+    int rc = check_nvt(st);
+    if (rc) return rc;
+    NEPX_REQUIRE(part >= NEPX_NVT_STEP_TWO && part <= NEPX_NVT_THERMO, NEPX_EINVAL, "bad integrator part %d", part);
+    NEPX_REQUIRE(st->dp, NEPX_EINVAL, "null dp buffer");
+    hipStream_t s = as_stream(stream);
+    const int nb = (st->n + kIB - 1) / kIB;
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part);
+    if (part == NEPX_NVT_THERMO)
+        hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1);
+    NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
 

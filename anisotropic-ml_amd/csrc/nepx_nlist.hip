@@ -393,12 +393,24 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
 }
 
 // ---- gather ----------------------------------------------------------------
+// 16 lanes per particle: lanes stride over the particle's forward row (i-role:
+// +F, +tau_i, +E) and reverse row (j-role: -F, +tau_j), accumulate in FP64 and
+// reduce with a fixed butterfly, so the result is independent of timing.
+constexpr int kGL = 16;
+
+__device__ __forceinline__ double red16(double v) {
+#pragma unroll
+    for (int o = kGL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kGL);
+    return v;
+}
+
 __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, const float4* __restrict__ out) {
-    const int k = blockIdx.x * kNB + threadIdx.x;
-    if (k >= st.n) return;
+    const int k = (blockIdx.x * kNB + threadIdx.x) / kGL;
+    const int sub = threadIdx.x % kGL;
+    if (k >= st.n) return;  // whole 16-lane groups exit together
     double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, u = 0;
     if (st.sys[k] >= 0 && !nl.flags[F_OVERFLOW]) {
-        for (int s = nl.row_start[k]; s < nl.row_start[k + 1]; ++s) {
+        for (int s = nl.row_start[k] + sub; s < nl.row_start[k + 1]; s += kGL) {
             const int c = nl.cidx[s];
             if (c < 0) continue;
             const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 1];
@@ -410,7 +422,7 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
             ty += b.y;
             tz += b.z;
         }
-        for (int e = nl.rrow_start[k]; e < nl.rrow_start[k + 1]; ++e) {
+        for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += kGL) {
             const int c = nl.cidx[nl.rnbr[e]];
             if (c < 0) continue;
             const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 2];
@@ -422,13 +434,22 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
             tz += b.z;
         }
     }
-    st.fx[k] = fx;
-    st.fy[k] = fy;
-    st.fz[k] = fz;
-    st.tx[k] = tx;
-    st.ty[k] = ty;
-    st.tz[k] = tz;
-    st.upot[k] = u;
+    fx = red16(fx);
+    fy = red16(fy);
+    fz = red16(fz);
+    tx = red16(tx);
+    ty = red16(ty);
+    tz = red16(tz);
+    u = red16(u);
+    if (sub == 0) {
+        st.fx[k] = fx;
+        st.fy[k] = fy;
+        st.fz[k] = fz;
+        st.tx[k] = tx;
+        st.ty[k] = ty;
+        st.tz[k] = tz;
+        st.upot[k] = u;
+    }
 }
 
 static int check_state(const nepx_state* st, const nepx_nlist* nl) {
@@ -539,7 +560,7 @@ extern "C" int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* p
     if (rc) return rc;
     NEPX_REQUIRE(pair_out && st->fx && st->fy && st->fz && st->tx && st->ty && st->tz && st->upot, NEPX_EINVAL,
                  "null force buffer");
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(st->n, kNB)), dim3(kNB), 0, as_stream(stream), *st, *nl,
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for((int64_t)st->n * kGL, kNB)), dim3(kNB), 0, as_stream(stream), *st, *nl,
                        (const float4*)pair_out);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;

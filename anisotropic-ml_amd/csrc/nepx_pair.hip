@@ -144,46 +144,77 @@ __device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool val
     }
 }
 
+// Networks whose packed weights fit this budget are staged in LDS once per block.
+constexpr int kWeightLdsBytes = 64 * 1024;
 template <class N>
-__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ W, int act, float en_min,
+constexpr bool weights_in_lds() {
+    return N::layout().total() * 4 <= kWeightLdsBytes;
+}
+template <class N>
+constexpr int weight_lds_floats() {
+    return weights_in_lds<N>() ? N::layout().total() : 4;
+}
+
+constexpr int kNetTiles = 2;  // 16-pair tiles sharing each A-operand read
+
+template <class N, int NT>
+__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, int act, float en_min,
                                                  float scale, int D, const float* __restrict__ G,
                                                  const int32_t* __restrict__ np_dev, int64_t maxp,
                                                  float* __restrict__ E, float* __restrict__ dEdG) {
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N>()];
+    const float* W = Wg;
+    if constexpr (weights_in_lds<N>()) {
+        stage_weights<N>(wl, Wg);
+        W = wl;
+    }
     const int P = n_pairs(np_dev, maxp);
     const int lane = threadIdx.x & 63;
-    const int nt = (P + 15) / 16;
+    const int nt = (P + 16 * NT - 1) / (16 * NT);
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * kBlock) >> 6;
     for (int tile = wave; tile < nt; tile += nwaves) {
-        const int p = tile * 16 + (lane & 15);
-        const bool valid = p < P;
-        v4f gin[N::MD];
-        gather_chunks<N>(G + (size_t)p * D, valid, D, gin);
-        v4f dg[N::MD];
-        const float y = ffnn_tile<N>(W, act, gin, scale, dg);
-        if (valid && lane < 16) E[p] = en_min + scale * y;
-        scatter_chunks<N>(dEdG + (size_t)p * D, valid, D, dg);
+        v4f gin[NT][N::MD];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int p = (tile * NT + t) * 16 + (lane & 15);
+            gather_chunks<N>(G + (size_t)p * D, p < P, D, gin[t]);
+        }
+        v4f dg[NT][N::MD];
+        float y[NT];
+        ffnn_tiles<N, NT>(W, act, gin, scale, dg, y);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int p = (tile * NT + t) * 16 + (lane & 15);
+            if (p < P && lane < 16) E[p] = en_min + scale * y[t];
+            scatter_chunks<N>(dEdG + (size_t)p * D, p < P, D, dg[t]);
+        }
     }
 }
 
 // Fused: descriptor -> LDS -> MFMA network -> LDS -> adjoint.  Each block
 // takes 256 consecutive pairs per iteration (64 per wave), so every wave of a
 // block runs the same number of iterations and the barriers are uniform.
-// Only the pair index survives from one phase to the next: the adjoint phase
-// reloads the pair and recomputes its forward accumulators, which keeps the
-// register footprint of the kernel at that of its largest phase.
+// The forward accumulators A of each lane's pair stay in registers through
+// the network phase and feed the adjoint directly.
 template <class Cfg, class N>
-__global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model m, const float* __restrict__ W,
-                                                  int act, float en_min, float scale,
-                                                  const float4* __restrict__ qf,
-                                                  const int32_t* __restrict__ pi,
-                                                  const int32_t* __restrict__ pj,
-                                                  const float4* __restrict__ pt,
-                                                  const int32_t* __restrict__ np_dev, int64_t maxp,
-                                                  float clip, float* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model m, const float* __restrict__ Wg,
+                                                                int act, float en_min, float scale,
+                                                                const float4* __restrict__ qf,
+                                                                const int32_t* __restrict__ pi,
+                                                                const int32_t* __restrict__ pj,
+                                                                const float4* __restrict__ pt,
+                                                                const int32_t* __restrict__ np_dev, int64_t maxp,
+                                                                float clip, float* __restrict__ out) {
     constexpr int LD = (Cfg::D + 3) / 4 * 4 + 4;  // row stride in LDS (16-byte aligned)
     __shared__ __attribute__((aligned(16))) float tile[kBlock * LD];
     __shared__ float etile[kBlock];
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N>()];
+    const float* W = Wg;
+    if constexpr (weights_in_lds<N>()) {
+        stage_weights<N>(wl, Wg);
+        W = wl;
+    }
     const int P = n_pairs(np_dev, maxp);
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -191,12 +222,12 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
     for (int base = blockIdx.x * kBlock; base < P; base += gridDim.x * kBlock) {
         const int idx = base + threadIdx.x;
         const bool valid = idx < P;
+        float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
+        if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+        const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        float A[Cfg::NANG + 1][Cfg::NLM];
         {
-            float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
-            if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
-            const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
             float grad[Cfg::NRAD + 1];
-            float A[Cfg::NANG + 1][Cfg::NLM];
             desc_accumulate<Cfg>(m, g, grad, A);
             float Gr[Cfg::D];
             desc_finish<Cfg>(grad, A, Gr);
@@ -205,32 +236,92 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
         }
         __syncthreads();
 #pragma unroll 1
-        for (int sub = 0; sub < 4; ++sub) {
-            const int local = wv * 64 + sub * 16 + (lane & 15);
-            float* row = tile + local * LD;
-            v4f gin[N::MD];
-            gather_chunks<N>(row, true, Cfg::D, gin);
-            v4f dg[N::MD];
-            const float y = ffnn_tile<N>(W, act, gin, scale, dg);
-            if (lane < 16) etile[local] = en_min + scale * y;
-            scatter_chunks<N>(row, true, Cfg::D, dg);
+        for (int sub = 0; sub < 64 / (16 * kNetTiles); ++sub) {
+            v4f gin[kNetTiles][N::MD];
+#pragma unroll
+            for (int tt = 0; tt < kNetTiles; ++tt) {
+                const int local = wv * 64 + (sub * kNetTiles + tt) * 16 + (lane & 15);
+                gather_chunks<N>(tile + local * LD, true, Cfg::D, gin[tt]);
+            }
+            v4f dg[kNetTiles][N::MD];
+            float y[kNetTiles];
+            ffnn_tiles<N, kNetTiles>(W, act, gin, scale, dg, y);
+#pragma unroll
+            for (int tt = 0; tt < kNetTiles; ++tt) {
+                const int local = wv * 64 + (sub * kNetTiles + tt) * 16 + (lane & 15);
+                if (lane < 16) etile[local] = en_min + scale * y[tt];
+                scatter_chunks<N>(tile + local * LD, true, Cfg::D, dg[tt]);
+            }
         }
         __syncthreads();
         if (valid) {
-            float4 qi, qj, t;
-            load_pair(qf, pi, pj, pt, idx, qi, qj, t);
-            const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
             float dG[Cfg::D];
 #pragma unroll
             for (int k = 0; k < Cfg::D; ++k) dG[k] = myrow[k];
-            float grad[Cfg::NRAD + 1];
-            float A[Cfg::NANG + 1][Cfg::NLM];
-            desc_accumulate<Cfg>(m, g, grad, A);
             const PairOut o = desc_adjoint<Cfg>(m, g, dG, A);
             store_out(out, idx, o, etile[threadIdx.x], clip);
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Reference-compat finite-difference stencil (PosToPtsKernels.py:153-446).
+// Translating i's sites by +dx e equals translating j by -dx e (the
+// descriptor depends only on the cloud up to translation); rotating i's sites
+// about X_i by dteta about e equals q_i -> dq q_i with dq = (cos dteta/2,
+// sin dteta/2 e); the rows in the frame of j rotate q_j the same way.
+__device__ __forceinline__ float4 qmul4(float4 a, float4 b) {  // (w, x, y, z) in (x, y, z, w) slots
+    return make_float4(a.x * b.x - (a.y * b.y + a.z * b.z + a.w * b.w),
+                       a.x * b.y + b.x * a.y + (a.z * b.w - a.w * b.z),
+                       a.x * b.z + b.x * a.z + (a.w * b.y - a.y * b.w),
+                       a.x * b.w + b.x * a.w + (a.y * b.z - a.z * b.y));
+}
+
+__global__ __launch_bounds__(kBlock) void k_stencil_rows(const float4* __restrict__ qf, const int32_t* __restrict__ pi,
+                                                         const int32_t* __restrict__ pj,
+                                                         const float4* __restrict__ pt, int P, float dx,
+                                                         float dteta, float4* __restrict__ qrows,
+                                                         int32_t* __restrict__ ri, int32_t* __restrict__ rj,
+                                                         float4* __restrict__ trows) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= 10 * P) return;
+    const int o = r / P, p = r % P;
+    float4 qi = qf[pi[p]], qj = qf[pj[p]], t = pt[p];
+    const float c = cosf(0.5f * dteta), s = sinf(0.5f * dteta);
+    if (o >= 1 && o <= 3) {
+        if (o == 1) t.x -= dx;
+        if (o == 2) t.y -= dx;
+        if (o == 3) t.z -= dx;
+    } else if (o >= 4) {
+        const int ax = (o - 4) % 3;
+        const float4 dq = make_float4(c, ax == 0 ? s : 0.f, ax == 1 ? s : 0.f, ax == 2 ? s : 0.f);
+        if (o <= 6) qi = qmul4(dq, qi);
+        else qj = qmul4(dq, qj);
+    }
+    qrows[2 * (size_t)r] = qi;
+    qrows[2 * (size_t)r + 1] = qj;
+    ri[r] = 2 * r;
+    rj[r] = 2 * r + 1;
+    trows[r] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fd_forces(const float* __restrict__ E, int P, float dx, float dteta,
+                                                      float clip, float* __restrict__ out) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= P) return;
+    const float e0 = E[p];
+    PairOut o;
+    o.fx = -(E[1 * (size_t)P + p] - e0) / dx;
+    o.fy = -(E[2 * (size_t)P + p] - e0) / dx;
+    o.fz = -(E[3 * (size_t)P + p] - e0) / dx;
+    o.tix = -(E[4 * (size_t)P + p] - e0) / dteta;
+    o.tiy = -(E[5 * (size_t)P + p] - e0) / dteta;
+    o.tiz = -(E[6 * (size_t)P + p] - e0) / dteta;
+    o.tjx = -(E[7 * (size_t)P + p] - e0) / dteta;
+    o.tjy = -(E[8 * (size_t)P + p] - e0) / dteta;
+    o.tjz = -(E[9 * (size_t)P + p] - e0) / dteta;
+    store_out(out, p, o, e0, clip);
 }
 
 // ---------------------------------------------------------------------------
@@ -244,6 +335,14 @@ using CfgThV1 = DescCfg<4, 10, 8, 4>;      // reference th_v1 with hypers 10 8 4
 
 using Net40x64 = NetCfg<48, 64, 2>;        // 40 -> 64 -> 64 -> 1
 using Net60x128 = NetCfg<64, 128, 2>;      // 60 -> 128 -> 128 -> 1
+using Net47x150 = NetCfg<48, 160, 4>;      // reference TheEnergyNet(150, 3, 47): 47 -> 150 x4 -> 1
+
+static int grid_for_cap(int64_t units, int per_block, int64_t cap) {
+    int64_t g = (units + per_block - 1) / per_block;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
 
 static int grid_for(int64_t maxp, int per_block) {
     int64_t g = (maxp + per_block - 1) / per_block;
@@ -357,12 +456,13 @@ extern "C" int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, co
     return NEPX_EUNSUP;
 }
 
-#define NEPX_NET_CASES(X)      \
-    X(40, 64, 1, Net40x64)     \
-    X(60, 128, 1, Net60x128)
+#define NEPX_NET_CASES(X)              \
+    X(40, 64, 1, Net40x64, kNetTiles)  \
+    X(60, 128, 1, Net60x128, 1)        \
+    X(47, 150, 3, Net47x150, 1)
 
 static bool net_supported(const nepx_net* n) {
-#define X(d, w, h, T) if (n->d_in == d && n->width == w && n->n_hidden == h) return true;
+#define X(d, w, h, T, NT) if (n->d_in == d && n->width == w && n->n_hidden == h) return true;
     NEPX_NET_CASES(X)
 #undef X
     return false;
@@ -376,13 +476,13 @@ extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int3
                  net->width, net->n_hidden + 1);
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-    const int grid = grid_for((maxp + 15) / 16, kBlock / 64);
-#define X(d, w, h, T)                                                                               \
-    if (net->d_in == d && net->width == w && net->n_hidden == h) {                                  \
-        hipLaunchKernelGGL(k_ffnn<T>, dim3(grid), dim3(kBlock), 0, as_stream(stream), net->w_packed, \
-                           net->act, net->en_min, scale, d, G, np_dev, maxp, E, dEdG);              \
-        NEPX_LAUNCH_CHECK();                                                                        \
-        return NEPX_OK;                                                                             \
+#define X(d, w, h, T, NT)                                                                              \
+    if (net->d_in == d && net->width == w && net->n_hidden == h) {                                     \
+        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64, 2 * 256);        \
+        hipLaunchKernelGGL((k_ffnn<T, NT>), dim3(grid), dim3(kBlock), 0, as_stream(stream),            \
+                           net->w_packed, net->act, net->en_min, scale, d, G, np_dev, maxp, E, dEdG);  \
+        NEPX_LAUNCH_CHECK();                                                                           \
+        return NEPX_OK;                                                                                \
     }
     NEPX_NET_CASES(X)
 #undef X
@@ -416,6 +516,17 @@ extern "C" int nepx_descriptor_adjoint(const nepx_pair_model* m, const float* qf
     X(10, 11, 6, 4, CfgTet10, 40, 64, 1, Net40x64) \
     X(12, 17, 6, 6, CfgCyl12, 60, 128, 1, Net60x128)
 
+extern "C" int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_net* net) {
+    if (!m || !net) return 0;
+#define X(ns, a, b, c, T, d, w, h, NT)                                                            \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&    \
+        net->width == w && net->n_hidden == h)                                                   \
+        return 1;
+    NEPX_FUSED_CASES(X)
+#undef X
+    return 0;
+}
+
 extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
                                const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
                                const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
@@ -426,7 +537,7 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && out, NEPX_EINVAL, "null buffer");
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-    const int grid = grid_for(maxp, kBlock);
+    const int grid = grid_for_cap(maxp, kBlock, 256);
 #define X(ns, a, b, c, T, d, w, h, NT)                                                                \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&         \
         net->width == w && net->n_hidden == h) {                                                      \
@@ -441,4 +552,30 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     set_error("no fused kernel for n_sites=%d hypers=(%d,%d,%d) net %d->%dx%d", m->n_sites, m->nrad,
               m->nang, m->lmax, net->d_in, net->width, net->n_hidden + 1);
     return NEPX_EUNSUP;
+}
+
+extern "C" int nepx_stencil_rows(const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                                 int64_t P, float dx, float dteta, float* q_rows, int32_t* row_i, int32_t* row_j,
+                                 float* t_rows, void* stream) {
+    NEPX_REQUIRE(P >= 0 && P <= (int64_t)1 << 27, NEPX_ECAP, "pair count %lld out of range", (long long)P);
+    if (P == 0) return NEPX_OK;
+    NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && q_rows && row_i && row_j && t_rows, NEPX_EINVAL, "null buffer");
+    const int rows = (int)(10 * P);
+    hipLaunchKernelGGL(k_stencil_rows, dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream),
+                       (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, (int)P, dx, dteta,
+                       (float4*)q_rows, row_i, row_j, (float4*)t_rows);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
+extern "C" int nepx_fd_forces(const float* E_rows, int64_t P, float dx, float dteta, float clip, float* out,
+                              void* stream) {
+    NEPX_REQUIRE(P >= 0 && P <= (int64_t)1 << 27, NEPX_ECAP, "pair count %lld out of range", (long long)P);
+    if (P == 0) return NEPX_OK;
+    NEPX_REQUIRE(E_rows && out, NEPX_EINVAL, "null buffer");
+    NEPX_REQUIRE(dx != 0.f && dteta != 0.f, NEPX_EINVAL, "zero finite-difference step");
+    hipLaunchKernelGGL(k_fd_forces, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream), E_rows,
+                       (int)P, dx, dteta, clip, out);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
 }

@@ -1,0 +1,172 @@
+"""DescriptorGenerator with the reference call surface (nep_simulator/DescriptorGenerator.py:13-442).
+
+generate_nep_descriptors(central_pos, orientations, Nlist) returns
+(G, pp, N_pair) like the reference, computed by the HIP kernels:
+  * Nlist (M, 2) candidate pairs (the reference's KD-tree list) are loaded
+    into the device list rows (nepx_nlist_from_pairs) when Nlist changes;
+  * the COM filter |mi(X_j - X_i)| < cutoff runs on the GPU in FP64 with a
+    deterministic compaction (nepx_pairs_filter), replacing
+    DescriptorGenerator.py:168-185;
+  * G has one row per pair (nepx_descriptor_fwd).  With fd_compat = True it
+    has the reference's 10 * N_pair stencil rows instead
+    (DescriptorGenerator.py:187-255), ordered op-major as the reference.
+The pair geometry of the last call stays on the device so ForceEvaluator can
+run the hand-written adjoint on it.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, ops
+from .model import make_pair_model, n_descriptor
+from .shapes import get_shape
+from .system import Arena, SystemSpec
+
+
+class DescriptorGenerator:
+    def set_timers(self):
+        self.t_pos_to_pts = 0.0
+        self.t_dx = 0.0
+        self.t_pts_to_nep = 0.0
+        self.t_nep1 = self.t_nep2 = self.t_nep3 = self.t_nep4 = self.t_nep5 = 0.0
+        self.is_sync = 0
+
+    def __init__(self):
+        self.set_timers()
+        self.fd_compat = False
+        self.device = torch.device("cuda:0")
+        self._ctx = None
+        self._nlist_key = None
+        self.dx = 1e-5
+        self.dteta = 1e-5
+
+    def setDevice(self, gpu_id):
+        self.set_timers()
+        if isinstance(gpu_id, torch.device):
+            self.device = gpu_id
+        else:
+            self.gpu_id = int(getattr(gpu_id, "id", gpu_id))
+            self.device = torch.device(f"cuda:{self.gpu_id}")
+
+    def setHyperParameters(self, hypers):
+        self.nrad = int(hypers[0])
+        self.nang = int(hypers[1])
+        self.lmax = int(hypers[2])
+        # hypers[3] is stored but, as in the reference, the descriptor cutoff comes from the shape
+        self.cutoff_nep = hypers[3] if len(hypers) > 3 else None
+        self.N_descriptors = n_descriptor(self.nrad, self.nang, self.lmax)
+        self.buffer_size = 0
+
+    def setSync(self, is_sync):
+        self.is_sync = is_sync
+
+    def getN_descriptor(self):
+        return self.N_descriptors
+
+    def setBoxSize(self, Lx):
+        self.Lx = float(Lx)
+
+    def set_dx_dteta(self, dx, dteta):
+        self.dx = float(dx)
+        self.dteta = float(dteta)
+
+    def setShape(self, shape_str):
+        """Shape registry (DescriptorGenerator.py:58-125); unknown names raise ValueError."""
+        self.shape = get_shape(shape_str)
+        self.pts_rep = self.shape.sites.copy()
+        self.Npts = self.shape.n_sites
+        self.Nd = 2 * self.Npts
+        self.en_min = self.shape.en_min
+        self.en_max = self.shape.en_max
+        self.cutoff = self.shape.cutoff
+        self.nep_cutoff = self.shape.nep_cutoff
+
+    def get_en_min_max(self):
+        return self.en_min, self.en_max
+
+    def move_data_to_device(self):
+        self.model = make_pair_model(self.shape, (self.nrad, self.nang, self.lmax))
+
+    # ------------------------------------------------------------------
+    def _context(self, n):
+        if self._ctx is None or self._ctx.specs[0].n != n or self._ctx.specs[0].L != self.Lx:
+            z3 = np.zeros((n, 3))
+            q = np.zeros((n, 4))
+            q[:, 0] = 1.0
+            spec = SystemSpec(z3, z3, q, np.zeros((n, 4)), 1.0, self.Lx, 1.0, 1e-3, (1.0, 1.0, 1.0), self.cutoff,
+                              skin=0.0)
+            self._ctx = Arena([spec], self.shape, (self.nrad, self.nang, self.lmax), None, device=self.device,
+                              list_cap=4096)
+            self._nlist_key = None
+        return self._ctx
+
+    def _sync(self):
+        if self.is_sync == 1:
+            torch.cuda.synchronize(self.device)
+
+    def generate_nep_descriptors(self, central_pos, orientations, Nlist):
+        self._sync()
+        t0 = time.time()
+        self.pos_to_pts(central_pos, orientations, Nlist)
+        self._sync()
+        t1 = time.time()
+        self.apply_dx_dteta()
+        self._sync()
+        t2 = time.time()
+        self.pts_to_nep_descriptors()
+        self._sync()
+        t3 = time.time()
+        self.t_pos_to_pts += t1 - t0
+        self.t_dx += t2 - t1
+        self.t_pts_to_nep += t3 - t2
+        return self.g_all_cupy, self.pp, self.N_pair
+
+    def pos_to_pts(self, central_pos, orientations, Nlist):
+        """Load the configuration, (re)load the candidate list, filter pairs within the COM cutoff."""
+        pos = torch.as_tensor(central_pos)
+        quat = torch.as_tensor(orientations)
+        ctx = self._context(pos.shape[0])
+        ctx.set_configuration(pos, quat)
+        nl = torch.as_tensor(Nlist)
+        key = (nl.data_ptr(), tuple(nl.shape), getattr(nl, "_version", 0))
+        if key != self._nlist_key:
+            ctx.load_pairs(nl)
+            self._nlist_key = key
+        ctx.filter_pairs()
+        self.N_pair = ctx.n_pairs()   # host read of the pair count, as len(self.pairs) in the reference
+        P = self.N_pair
+        self.pair_i = ctx.lt["pair_i"][:P]
+        self.pair_j = ctx.lt["pair_j"][:P]
+        self.pair_t = ctx.lt["pair_t"][: 4 * P].view(P, 4)
+        self.pp = torch.stack([self.pair_i, self.pair_j], 1).to(torch.int64)
+        self.pairs = self.pp
+
+    def apply_dx_dteta(self):
+        """Build the 10-row FD stencil inputs when fd_compat is on (no-op for the analytic path)."""
+        if not self.fd_compat:
+            return
+        P = self.N_pair
+        ctx = self._ctx
+        dev = self.device
+        self.q_rows = torch.empty((20 * P, 4), dtype=torch.float32, device=dev)
+        self.row_i = torch.empty(10 * P, dtype=torch.int32, device=dev)
+        self.row_j = torch.empty(10 * P, dtype=torch.int32, device=dev)
+        self.t_rows = torch.empty((10 * P, 4), dtype=torch.float32, device=dev)
+        _lib.check(_lib.lib.nepx_stencil_rows(
+            _lib.ptr(ctx.t["qf"]), _lib.ptr(self.pair_i), _lib.ptr(self.pair_j), _lib.ptr(self.pair_t.contiguous()),
+            P, self.dx, self.dteta, _lib.ptr(self.q_rows), _lib.ptr(self.row_i), _lib.ptr(self.row_j),
+            _lib.ptr(self.t_rows), _lib.stream_handle()), "nepx_stencil_rows")
+
+    def pts_to_nep_descriptors(self):
+        ctx = self._ctx
+        if self.fd_compat:
+            self.g_all_cupy = ops.descriptor_forward(self.model, self.q_rows, self.row_i, self.row_j, self.t_rows)
+        else:
+            self.g_all_cupy = ops.descriptor_forward(self.model, ctx.t["qf"].view(-1, 4), self.pair_i.contiguous(),
+                                                     self.pair_j.contiguous(), self.pair_t.contiguous())
+        return self.g_all_cupy
+
+
+def dummy():
+    pass

@@ -32,7 +32,7 @@ class Net(C.Structure):
 
 _STATE_PTRS = ["x", "y", "z", "vx", "vy", "vz", "q0", "q1", "q2", "q3", "p0", "p1", "p2", "p3", "mass",
                "fx", "fy", "fz", "tx", "ty", "tz", "upot", "xb", "yb", "zb", "qf", "sys", "sp", "th",
-               "partials"]
+               "partials", "dp"]
 
 
 class State(C.Structure):
@@ -59,6 +59,7 @@ THERMO_DTYPE = np.dtype([("xi", "f8"), ("eta", "f8"), ("xi_r", "f8"), ("eta_r", 
                          ("pad", "f8", 6)])
 
 FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW = range(6)
+NVT_STEP_TWO, NVT_TRANSLATE, NVT_ROTATE, NVT_THERMO = 2, 3, 4, 5
 N_FLAGS = 8
 
 
@@ -85,12 +86,16 @@ def _load():
                                      vp, vp], C.c_int),
         "nepx_pair_fused": ([C.POINTER(PairModel), C.POINTER(Net), vp, vp, vp, vp, vp, C.c_int64, C.c_float,
                              vp, vp], C.c_int),
+        "nepx_pair_fused_supported": ([C.POINTER(PairModel), C.POINTER(Net)], C.c_int),
         "nepx_nlist_build": ([C.POINTER(State), C.POINTER(NList), C.c_int32, vp], C.c_int),
         "nepx_nlist_from_pairs": ([C.POINTER(State), C.POINTER(NList), vp, C.c_int64, vp], C.c_int),
         "nepx_pairs_filter": ([C.POINTER(State), C.POINTER(NList), vp], C.c_int),
         "nepx_forces_gather": ([C.POINTER(State), C.POINTER(NList), vp, vp], C.c_int),
         "nepx_nvt_step": ([C.POINTER(State), C.POINTER(NList), vp], C.c_int),
-        "nepx_nvt_init": ([C.POINTER(State), C.POINTER(NList), vp], C.c_int),
+        "nepx_nvt_init": ([C.POINTER(State), C.POINTER(NList), C.c_int32, vp], C.c_int),
+        "nepx_nvt_part": ([C.POINTER(State), C.POINTER(NList), C.c_int32, vp], C.c_int),
+        "nepx_stencil_rows": ([vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, vp, vp, vp, vp, vp], C.c_int),
+        "nepx_fd_forces": ([vp, C.c_int64, C.c_float, C.c_float, C.c_float, vp, vp], C.c_int),
         "nepx_md_step": ([C.POINTER(State), C.POINTER(NList), C.POINTER(PairModel), C.POINTER(Net), C.c_float,
                           vp, vp], C.c_int),
     }

@@ -23,9 +23,9 @@ class StepRunner:
         self._ev = None
         self._ev_i = 0
 
-    def init(self):
+    def init(self, update_thermostat=True):
         with torch.cuda.stream(self.stream):
-            self.ar.init(self.stream)
+            self.ar.init(self.stream, update_thermostat)
         self.stream.synchronize()
         if self.use_graph:
             self._capture()
@@ -43,8 +43,8 @@ class StepRunner:
         self.ar.nvt_step(self.stream)
 
     def _capture(self):
-        # warm the segments once eagerly (one real step), then capture
-        self._eager_step()
+        # buffers and kernel choice are settled before capture; capture launches nothing
+        self.ar.prepare_pair()
         self.stream.synchronize()
         self.graphs = []
         for seg in (self._pre, self._pair, self._post):

@@ -64,11 +64,11 @@ class Arena:
         self.shape = shape
         self.hypers = tuple(int(h) for h in hypers[:3])
         self.D = n_descriptor(*self.hypers)
-        if net.d_in != self.D:
+        if net is not None and net.d_in != self.D:
             raise ValueError(f"network input {net.d_in} != descriptor size {self.D}")
         self.net = net
         self.model = make_pair_model(shape, self.hypers, r_nep)
-        self.cnet = net.cstruct()
+        self.cnet = net.cstruct() if net is not None else None
         self.clip = float(clip)
         self._layout()
         self._alloc_state()
@@ -138,6 +138,7 @@ class Arena:
             th[k]["e_r"] = 1.0
         self.t["th"] = torch.from_numpy(th.view(np.uint8).copy()).to(dev)
         self.t["partials"] = torch.zeros(4 * (n // BLOCK + 1), dtype=torch.float64, device=dev)
+        self.t["dp"] = torch.zeros(4 * n, dtype=torch.float64, device=dev)
         st = _lib.State()
         st.n, st.n_sys = n, len(self.specs)
         for k in _lib._STATE_PTRS:
@@ -200,16 +201,21 @@ class Arena:
                 raise _lib.NepxError(f"neighbour list overflow persists (flags {f.tolist()})")
         return int(self.flags()[_lib.FLAG_PL])
 
-    def init(self, stream=None):
-        """First list build (with capacity check) and initial kinetic energies + thermostat update."""
+    def init(self, stream=None, update_thermostat=True):
+        """First list build (with capacity check), initial kinetic energies and, unless the
+        thermostat state was restored from a saved run, the first thermostat update."""
         self.build_list(True, stream)
         self.ensure_capacity()
-        _lib.check(_lib.lib.nepx_nvt_init(C.byref(self.st), C.byref(self.nl), self._s(stream)), "nepx_nvt_init")
+        self.nvt_init(stream, update_thermostat)
 
     def step(self, stream=None):
-        _lib.check(_lib.lib.nepx_md_step(C.byref(self.st), C.byref(self.nl), C.byref(self.model),
-                                         C.byref(self.cnet), self.clip, _lib.ptr(self.pair_out),
-                                         self._s(stream)), "nepx_md_step")
+        """One MD step, device only: [list rebuild if the skin is used up] -> filter -> pair kernels ->
+        gather -> NVT (Sim.step, Sim.py:257-285)."""
+        self.build_list(False, stream)
+        self.filter_pairs(stream)
+        self.pair_fused(stream)
+        self.gather(stream)
+        self.nvt_step(stream)
         self.steps += 1
 
     def check(self):
@@ -236,14 +242,93 @@ class Arena:
                                                self._s(stream)), "nepx_forces_gather")
 
     def pair_fused(self, stream=None):
+        """Per-pair forces/torques/energies of the compacted pairs.
+
+        One fused kernel when an instance exists for this (shape, hypers,
+        network); otherwise the three HIP kernels descriptor -> network ->
+        adjoint with G and dE/dG in HBM.
+        """
         np_dev = C.c_void_p(self.lt["flags"].data_ptr() + 4 * _lib.FLAG_P)
-        _lib.check(_lib.lib.nepx_pair_fused(C.byref(self.model), C.byref(self.cnet), _lib.ptr(self.t["qf"]),
-                                            _lib.ptr(self.lt["pair_i"]), _lib.ptr(self.lt["pair_j"]),
-                                            _lib.ptr(self.lt["pair_t"]), np_dev, self.cap, self.clip,
-                                            _lib.ptr(self.pair_out), self._s(stream)), "nepx_pair_fused")
+        args = (_lib.ptr(self.t["qf"]), _lib.ptr(self.lt["pair_i"]), _lib.ptr(self.lt["pair_j"]),
+                _lib.ptr(self.lt["pair_t"]), np_dev, self.cap)
+        self.prepare_pair()
+        if not self.split:
+            _lib.check(_lib.lib.nepx_pair_fused(C.byref(self.model), C.byref(self.cnet), *args, self.clip,
+                                                _lib.ptr(self.pair_out), self._s(stream)), "nepx_pair_fused")
+            return
+        s = self._s(stream)
+        _lib.check(_lib.lib.nepx_descriptor_fwd(C.byref(self.model), *args, _lib.ptr(self._G), s),
+                   "nepx_descriptor_fwd")
+        _lib.check(_lib.lib.nepx_ffnn_fwd_bwd(C.byref(self.cnet), _lib.ptr(self._G), np_dev, self.cap,
+                                              _lib.ptr(self._E), _lib.ptr(self._dG), s), "nepx_ffnn_fwd_bwd")
+        _lib.check(_lib.lib.nepx_descriptor_adjoint(C.byref(self.model), *args, _lib.ptr(self._dG),
+                                                    _lib.ptr(self._E), self.clip, _lib.ptr(self.pair_out), s),
+                   "nepx_descriptor_adjoint")
+
+    def prepare_pair(self):
+        """Choose fused or split pair kernels and allocate the split path's buffers (no launches)."""
+        if getattr(self, "split", None) is None:
+            self.split = not _lib.lib.nepx_pair_fused_supported(C.byref(self.model), C.byref(self.cnet))
+        if self.split and (getattr(self, "_G", None) is None or self._G.numel() < self.cap * self.D):
+            self._G = torch.empty(self.cap * self.D, dtype=torch.float32, device=self.device)
+            self._dG = torch.empty(self.cap * self.D, dtype=torch.float32, device=self.device)
+            self._E = torch.empty(self.cap, dtype=torch.float32, device=self.device)
 
     def nvt_step(self, stream=None):
         _lib.check(_lib.lib.nepx_nvt_step(C.byref(self.st), C.byref(self.nl), self._s(stream)), "nepx_nvt_step")
+
+    def nvt_part(self, part, stream=None):
+        _lib.check(_lib.lib.nepx_nvt_part(C.byref(self.st), C.byref(self.nl), int(part), self._s(stream)),
+                   "nepx_nvt_part")
+
+    def nvt_init(self, stream=None, update_thermostat=True):
+        _lib.check(_lib.lib.nepx_nvt_init(C.byref(self.st), C.byref(self.nl), int(bool(update_thermostat)),
+                                          self._s(stream)), "nepx_nvt_init")
+
+    def load_pairs(self, pairs, stream=None):
+        """Use an explicit (M, 2) candidate list (any order) as the list rows."""
+        pairs = pairs.to(device=self.device, dtype=torch.int64).contiguous()
+        m = pairs.shape[0]
+        if m + 1024 > self.cap:
+            self._alloc_list(m + 4096)
+        _lib.check(_lib.lib.nepx_nlist_from_pairs(C.byref(self.st), C.byref(self.nl), _lib.ptr(pairs), m,
+                                                  self._s(stream)), "nepx_nlist_from_pairs")
+        f = self.flags()
+        if f[_lib.FLAG_OVERFLOW]:
+            raise _lib.NepxError(f"bad candidate pair list (code {int(f[_lib.FLAG_OVERFLOW])}): indices out of "
+                                 "range, i == j, or a row longer than 2048")
+
+    def set_configuration(self, x=None, q=None, k=0):
+        """Overwrite positions (N,3) and/or orientations (N,4) of system k from tensors/arrays."""
+        a, b = self.starts[k], self.starts[k] + self.specs[k].n
+        dev = self.device
+        if x is not None:
+            x = torch.as_tensor(x).to(device=dev, dtype=torch.float64)
+            for c, key in enumerate("xyz"):
+                self.t[key][a:b].copy_(x[:, c])
+        if q is not None:
+            q = torch.as_tensor(q).to(device=dev, dtype=torch.float64)
+            for c in range(4):
+                self.t[f"q{c}"][a:b].copy_(q[:, c])
+            self.t["qf"].view(-1, 4)[a:b].copy_(q.to(torch.float32))
+
+    def set_forces(self, F, T, k=0):
+        a, b = self.starts[k], self.starts[k] + self.specs[k].n
+        F = torch.as_tensor(F).to(device=self.device, dtype=torch.float64)
+        T = torch.as_tensor(T).to(device=self.device, dtype=torch.float64)
+        for c, key in enumerate("xyz"):
+            self.t["f" + key][a:b].copy_(F[:, c])
+            self.t["t" + key][a:b].copy_(T[:, c])
+
+    def forces(self, k=0):
+        a, b = self.starts[k], self.starts[k] + self.specs[k].n
+        F = torch.stack([self.t["fx"][a:b], self.t["fy"][a:b], self.t["fz"][a:b]], 1)
+        T = torch.stack([self.t["tx"][a:b], self.t["ty"][a:b], self.t["tz"][a:b]], 1)
+        return F, T
+
+    def tensor(self, keys, k=0):
+        a, b = self.starts[k], self.starts[k] + self.specs[k].n
+        return torch.stack([self.t[key][a:b] for key in keys], 1)
 
     # --------------------------------------------------------------- access
     def thermo(self):

@@ -84,6 +84,7 @@ typedef struct nepx_state {
     nepx_sysparam* sp;               /* n_sys records                             */
     nepx_thermo* th;                 /* n_sys records                             */
     double* partials;                /* 4 doubles per integrator block            */
+    double* dp;                      /* 4 doubles per particle: torque kick of step two */
 } nepx_state;
 
 /* Verlet half list (i<j, rows sorted by j) + reverse rows + per-step compaction. */
@@ -142,6 +143,23 @@ int nepx_descriptor_adjoint(const nepx_pair_model* m, const float* qf, const int
                             const int32_t* pair_j, const float* pair_t, const int32_t* n_pairs_dev,
                             int64_t max_pairs, const float* dEdG, const float* E, float clip,
                             float* out, void* stream);
+/* Reference-compat finite differences (DescriptorGenerator.py:220-255,
+ * PosToPtsKernels.py:153-446): expand P pairs into the 10 stencil rows
+ * (row o*P+p: o=0 unperturbed, 1-3 i's sites + dx e_x/e_y/e_z, 4-6 i's sites
+ * rotated by dteta about x/y/z through X_i, 7-9 j's sites rotated about X_j).
+ * Writes row orientations q_rows (20P x 4: i then j of each row), row pair
+ * indices (2r, 2r+1) and row translations t_rows (10P x 4), for nepx_descriptor_fwd. */
+int nepx_stencil_rows(const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                      int64_t n_pairs, float dx, float dteta, float* q_rows, int32_t* row_i, int32_t* row_j,
+                      float* t_rows, void* stream);
+/* Forward-difference forces/torques from the 10P row energies, clipped at
+ * +-clip (ForceEvaluator.py:86-132).  out: P x 12 floats as the adjoint writes. */
+int nepx_fd_forces(const float* E_rows, int64_t n_pairs, float dx, float dteta, float clip, float* out,
+                   void* stream);
+
+/* 1 when nepx_pair_fused has a kernel for this pair model and network, else 0
+ * (callers then run nepx_descriptor_fwd -> nepx_ffnn_fwd_bwd -> nepx_descriptor_adjoint). */
+int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_net* net);
 /* Fused descriptor -> MFMA network -> adjoint for the step (no G in HBM). */
 int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
                     const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
@@ -169,9 +187,21 @@ int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* pair_out, vo
  * (NVT.py:17-129) in FP64, then per-system kinetic energies, thermostat
  * update for the next step (update_variables) and the list rebuild test. */
 int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream);
-/* Kinetic energies of the current state and the thermostat update that the
- * reference performs at the start of the first step (Sim.py:147-169). */
-int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, void* stream);
+/* Kinetic energies of the current state and, when update_thermostat != 0, the
+ * thermostat update that the reference performs at the start of the first step
+ * (Sim.py:147-169).  A run resumed from a saved thermostat state passes 0. */
+int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_thermostat, void* stream);
+/* One part of the step, for callers that drive the reference's methods one by one:
+ * NEPX_NVT_STEP_TWO  = integrate_step_two minus the thermostat update (NVT.py:17-64)
+ * NEPX_NVT_TRANSLATE = translate1 (NVT.py:67-85)
+ * NEPX_NVT_ROTATE    = rotate1 (NVT.py:87-106), using the kick stored by step two
+ * NEPX_NVT_THERMO    = advance_thermostat1 (NVT.py:108-129) + the thermostat update
+ * Running the four in order equals nepx_nvt_step. */
+#define NEPX_NVT_STEP_TWO 2
+#define NEPX_NVT_TRANSLATE 3
+#define NEPX_NVT_ROTATE 4
+#define NEPX_NVT_THERMO 5
+int nepx_nvt_part(nepx_state* st, nepx_nlist* nl, int32_t part, void* stream);
 
 /* One full MD step: [list rebuild if needed] -> filter -> fused pair kernel ->
  * gather -> NVT.  Replaces Sim.step (Sim.py:257-285).  pair_out: cap x 12 floats. */

@@ -5,6 +5,9 @@ Input: 64 pair site clouds sampled (seed 0) from the reference's data file
 with numpy's safe loader).  Expected output: the FP64 oracle descriptor with
 the cube_v2 model hypers (10, 8, 4) and r_nep = 4.5
 (nep_simulator/DescriptorGenerator.py:63-79, runner.sh:2).
+Also copies the reference's initial configuration
+nep_simulator/init/th_init_N200.gsd (200 tetrahedra, box 33, written by
+HOOMD-blue 4.7.0) unchanged as tests/golden/th_init_N200.gsd.
 Run in a container that holds the reference checkout:
     python tests/golden/make_fixtures.py
 """
@@ -18,6 +21,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from oracle import nep_oracle as O  # noqa: E402
 
 SRC = "/root/reference/01/datadir/x_12pts_500_s42.npy"
+INIT = "/root/reference/nep_simulator/init/th_init_N200.gsd"
 
 if __name__ == "__main__":
     x = np.load(SRC, allow_pickle=False)
@@ -26,3 +30,5 @@ if __name__ == "__main__":
     G = O.descriptor(clouds.astype(np.float64), 4.5, 10, 8, 4)
     np.savez(os.path.join(HERE, "x12pts_cube_v2_sample.npz"), clouds=clouds, G=G, index=idx)
     print("wrote", clouds.shape, G.shape)
+    import shutil
+    shutil.copyfile(INIT, os.path.join(HERE, "th_init_N200.gsd"))

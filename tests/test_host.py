@@ -1,0 +1,163 @@
+"""Host-side checks (no GPU): the C ABI library, weight packing, GSD I/O, shapes and workloads."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import nep_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    from nepx import _lib
+    hdr = open(os.path.join(ROOT, "include", "nepx.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(nepx_\w+)\(", hdr, re.M)))
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(_lib.lib, n)]
+    assert not missing, missing
+    assert _lib.lib.nepx_version() >= 1
+
+
+def test_struct_layouts_match_binding():
+    from nepx import _lib
+    sizes = (C.c_int32 * 8)()
+    assert _lib.lib.nepx_struct_sizes(sizes) == 0
+    assert list(sizes[:6]) == [C.sizeof(_lib.PairModel), C.sizeof(_lib.Net), _lib.SYSPARAM_DTYPE.itemsize,
+                               _lib.THERMO_DTYPE.itemsize, C.sizeof(_lib.State), C.sizeof(_lib.NList)]
+
+
+def _emulate_mfma_net(packed, d_in, width, n_hidden, G, act):
+    """Run the packed operands through the documented 16x16x4 lane layout on the CPU.
+
+    A operand lane l: A[l%16][l/16]; B operand lane l: B[l/16][l%16];
+    accumulator lane l: rows 4*(l/16)+r, column l%16.  Any mistake in the
+    packing or in the K-permutation shows up as a wrong network output.
+    """
+    DP, HP, NL = (d_in + 15) // 16 * 16, (width + 15) // 16 * 16, n_hidden + 1
+
+    def fwd_size(k):
+        return (HP // 16) * ((DP if k == 0 else HP) // 4) * 64
+
+    def bwd_size(k):
+        return ((DP if k == 0 else HP) // 16) * (HP // 4) * 64
+
+    fwd_off = [sum(fwd_size(j) for j in range(k)) for k in range(NL)]
+    bwd0 = sum(fwd_size(j) for j in range(NL))
+    bwd_off = [bwd0 + sum(bwd_size(j) for j in range(k)) for k in range(NL)]
+    bias_off = [bwd_off[-1] + bwd_size(NL - 1) + k * HP for k in range(NL)]
+    wout = bias_off[-1] + HP
+    bout = wout + HP
+    P = G.shape[0]
+    Gp = np.zeros((P, DP))
+    Gp[:, :d_in] = G
+
+    def matmul(Wsrc, off, M, K, X):          # returns (M, P) = W (M x K) . X (K x P) with the lane permutation
+        out = np.zeros((M, X.shape[1]))
+        for mt in range(M // 16):
+            for s in range(K // 4):
+                for lane in range(64):
+                    a = Wsrc[off + (mt * (K // 4) + s) * 64 + lane]
+                    row = 16 * mt + lane % 16
+                    kidx = 16 * (s // 4) + 4 * (lane // 16) + s % 4
+                    out[row] += a * X[kidx]
+        return out
+
+    h = Gp.T
+    ders = []
+    for k in range(NL):
+        K = DP if k == 0 else HP
+        z = matmul(packed, fwd_off[k], HP, K, h) + packed[bias_off[k]:bias_off[k] + HP][:, None]
+        h = np.tanh(z) if act == "tanh" else np.maximum(z, 0)
+        ders.append(1 - h * h if act == "tanh" else (h > 0).astype(float))
+    y = packed[wout:wout + HP] @ h + packed[bout]
+    delta = packed[wout:wout + HP][:, None] * ders[-1]
+    for k in range(NL - 1, 0, -1):
+        delta = matmul(packed, bwd_off[k], HP, HP, delta) * ders[k - 1]
+    dG = matmul(packed, bwd_off[0], DP, HP, delta)
+    return y, dG[:d_in].T
+
+
+@pytest.mark.parametrize("d_in,width,n_hidden,act", [(40, 64, 1, "tanh"), (47, 150, 3, "relu"), (60, 128, 1, "tanh")])
+def test_net_packing_reproduces_dense_network(d_in, width, n_hidden, act):
+    import nepx
+    net = nepx.EnergyNet.random(d_in, [width] * (n_hidden + 1), seed=5, act=act, device=None)
+    rng = np.random.default_rng(0)
+    G = rng.uniform(0, 2, (5, d_in))
+    y, dG = _emulate_mfma_net(net.packed.astype(np.float64), d_in, width, n_hidden, G, act)
+    y64, g64 = O.ffnn_forward_backward(G, net.layers_f64(), act)
+    assert np.allclose(y, y64, rtol=1e-5, atol=1e-5)
+    assert np.allclose(dG, g64, rtol=1e-4, atol=1e-5)
+
+
+def test_net_pack_rejects_bad_input():
+    from nepx import _lib
+    assert _lib.lib.nepx_net_pack(0, 64, 1, None, None, None) == -1
+    assert b"bad network dims" in _lib.lib.nepx_last_error()
+
+
+def test_gsd_reads_reference_init_and_round_trips(tmp_path):
+    from nepx import gsd
+    fr = gsd.read_frame(os.path.join(ROOT, "tests", "golden", "th_init_N200.gsd"))
+    assert int(fr["particles/N"][0]) == 200
+    assert np.allclose(fr["configuration/box"][:3], 33.0)
+    assert fr["particles/orientation"].shape == (200, 4)
+    assert np.allclose(np.linalg.norm(fr["particles/orientation"], axis=1), 1.0, atol=1e-5)
+    assert np.all(fr["particles/mass"] == 1.0) and np.all(fr["particles/typeid"] == 0)   # schema defaults
+    path = str(tmp_path / "t.gsd")
+    w = gsd.GsdWriter(path, "w")
+    x = np.random.default_rng(1).random((7, 3))
+    for step in (0, 5):
+        w.append({"configuration/step": np.array([step], np.uint64), "particles/N": np.array([7], np.uint32),
+                  "log/nepx/position_f64": x + step})
+    assert gsd.n_frames(path) == 2
+    back = gsd.read_frame(path, 1)
+    assert np.array_equal(back["log/nepx/position_f64"], x + 5) and int(back["configuration/step"][0]) == 5
+    assert np.array_equal(gsd.read_frame(path, 0)["log/nepx/position_f64"], x)
+
+
+def test_gsd_rejects_non_gsd(tmp_path):
+    from nepx import gsd
+    p = tmp_path / "bad.gsd"
+    p.write_bytes(b"\0" * 300)
+    with pytest.raises(gsd.GsdError):
+        gsd.read_frame(str(p))
+
+
+def test_product_shapes_equal_oracle_shapes():
+    import nepx
+    T = O.shape_table()
+    for name, sh in T.items():
+        s = nepx.get_shape(name)
+        assert np.allclose(s.sites, sh["sites"]) and s.cutoff == sh["cutoff"] and s.nep_cutoff == sh["nep_cutoff"]
+        assert (s.en_min, s.en_max) == (sh["en_min"], sh["en_max"])
+    with pytest.raises(ValueError):
+        nepx.get_shape("banana")
+
+
+def test_workload_geometry():
+    from nepx.configs import make_system
+    s = make_system("tet10", 1000, 0.45, seed=1)
+    assert np.isclose(s.L, (1000 * (1 / (6 * np.sqrt(2))) / 0.45) ** (1 / 3))
+    assert np.all(np.abs(s.x) <= 0.5 * s.L)
+    assert np.allclose(np.linalg.norm(s.q, axis=1), 1.0)
+    # p = 2 q (x) (0, L): the body angular momentum recovered by s = q* (x) p / 2 has no scalar part
+    sq = 0.5 * O.quat_mul(O.quat_conj(s.q), s.p)
+    assert np.allclose(sq[:, 0], 0.0, atol=1e-12)
+    c = make_system("cube8", 64, 0.3, seed=2)
+    from nepx.configs import _rotmat, cubes_overlap
+    for i in range(64):
+        for j in range(i + 1, 64):
+            t = O.min_image(c.x[j] - c.x[i], c.L)
+            if np.linalg.norm(t) < np.sqrt(3):
+                assert not cubes_overlap(t, _rotmat(c.q[i]), _rotmat(c.q[j]))
+
+
+def test_bench_refuses_multi_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stdout
