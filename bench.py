@@ -22,6 +22,17 @@ PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
+def profiled_traffic(workload):
+    """HBM bytes per launch of the fused pair kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy), or None when no summary exists."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"k_fused_{workload}_hbm_counters.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
 def ffnn_flops_per_pair(D, widths):
     h1, h2 = widths
     return 4 * (D * h1 + h1 * h2 + h2)
@@ -73,6 +84,9 @@ def main():
     ap.add_argument("--workload", default="tet32k")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="capture the step in a hipGraph (1) or launch eagerly (0)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="after the timed region, copy MIB MiB device-to-device once (a kernel of known traffic "
+                         "for scaling profiler byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "nepx runs on one GPU per process; multi-GPU runs are not part of this build "
@@ -122,6 +136,7 @@ def main():
 
     flops = ffnn_flops_per_pair(D, widths) * P
     achieved = flops / (fused_ms * 1e-3) / 1e12
+    traffic, traffic_src = profiled_traffic(a.workload)
     res = {
         "metric": "particle-steps/s at 32k tetrahedra",
         "value": spec.n / (ms_step * 1e-3),
@@ -140,11 +155,19 @@ def main():
                    "particles": spec.n, "pairs_per_step": P, "hipgraph": bool(a.graph),
                    "list_builds": int(flags[4]), "wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_fused (descriptor + FFNN fwd/bwd + adjoint)", "kernel_ms": fused_ms,
                      "flops_per_launch": flops},
         "thermo": {"Kt_per_dof": float(th["Kt"] / spec.ndof * 2), "U": float(th["U"])},
     }
+    if a.copy_calib:
+        n = a.copy_calib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        res["copy_calib_bytes"] = {"read": 4 * n, "write": 4 * n}
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(spec, w["shape"], hyp, net)
     print(json.dumps(res))
