@@ -89,7 +89,7 @@ enum { M_STEP = 0, M_KE = 1, M_TWO = 2, M_TRANS = 3, M_ROT = 4, M_THERMO = 5 };
 __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
     const int i = blockIdx.x * kIB + threadIdx.x;
     double kt = 0.0, kr = 0.0, u = 0.0, d2 = 0.0;
-    if (i < st.n && st.sys[i] >= 0) {
+    if (i < st.n && st.sys[i] >= 0 && st.sp[st.sys[i]].active) {
         const int sidx = st.sys[i];
         const nepx_sysparam& sp = st.sp[sidx];
         const nepx_thermo& th = st.th[sidx];
@@ -179,9 +179,11 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
 
 // One block per system: fixed-order sums of the block partials, thermostat
 // update, list rebuild test.
-__global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* flags, int update) {
+__global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* flags, int update, int count_step) {
     const int s = blockIdx.x;
-    const nepx_sysparam& sp = st.sp[s];
+    nepx_sysparam& sp = st.sp[s];
+    if (!sp.active) return;  // uniform per block
+    if (update == 2 && st.th[s].pending == 0.0) return;  // only newly admitted systems
     double a = 0, b = 0, c = 0, d = 0;
     for (int k = sp.blk_start + threadIdx.x; k < sp.blk_end; k += kIB) {
         const double* pp = st.partials + 4 * (size_t)k;
@@ -200,6 +202,7 @@ __global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* f
         th.Kr = Kr;
         th.U = out[2];
         th.max_disp2 = out[3];
+        th.pending = 0.0;
         if (update) {
             const double dt = sp.dt, tau = sp.tau, kT = sp.kT;
             const double cfac = 0.5 * ((dt / tau) / tau);
@@ -221,6 +224,10 @@ __global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* f
         }
         const double half = 0.5 * sp.skin;
         if (flags && out[3] > half * half) flags[0] = 1;
+        if (count_step) {
+            th.steps += 1.0;
+            if (th.budget > 0.0 && th.steps >= th.budget) sp.active = 0;  // finished: frozen until harvested
+        }
     }
 }
 
@@ -242,7 +249,7 @@ extern "C" int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream) {
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
     hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0);
-    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
@@ -254,7 +261,7 @@ extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_ther
     const int nb = (st->n + kIB - 1) / kIB;
     hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1);
     hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr,
-                       update_thermostat ? 1 : 0);
+                       update_thermostat == 2 ? 2 : (update_thermostat ? 1 : 0), 0);
     NEPX_LAUNCH_CHECK();
     (void)nl;
     return NEPX_OK;
@@ -269,7 +276,7 @@ extern "C" int nepx_nvt_part(nepx_state* st, nepx_nlist* nl, int32_t part, void*
     const int nb = (st->n + kIB - 1) / kIB;
     hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part);
     if (part == NEPX_NVT_THERMO)
-        hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1);
+        hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }

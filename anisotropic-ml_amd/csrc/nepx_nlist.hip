@@ -42,6 +42,12 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc) {
 
 __device__ __forceinline__ bool want_build(const int32_t* flags) { return flags[F_REBUILD] != 0; }
 
+// A particle takes part in the list when it belongs to an active system.
+__device__ __forceinline__ bool live(const nepx_state& st, int i) {
+    const int s = st.sys[i];
+    return s >= 0 && st.sp[s].active;
+}
+
 __global__ void k_set_flag(int32_t* flags, int idx, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) flags[idx] = v;
 }
@@ -55,9 +61,8 @@ __global__ void k_zero_i32(int32_t* a, int64_t n, const int32_t* flags) {
 __global__ void k_cells_count(nepx_state st, nepx_nlist nl) {
     if (!want_build(nl.flags)) return;
     const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i >= st.n) return;
+    if (i >= st.n || !live(st, i)) return;
     const int s = st.sys[i];
-    if (s < 0) return;
     const nepx_sysparam& sp = st.sp[s];
     if (sp.nc <= 0) return;
     const int cx = cell_coord(st.x[i], sp.L, sp.nc), cy = cell_coord(st.y[i], sp.L, sp.nc),
@@ -100,9 +105,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_cells(nepx_nlist nl) {
 __global__ void k_cells_fill(nepx_state st, nepx_nlist nl) {
     if (!want_build(nl.flags)) return;
     const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i >= st.n) return;
+    if (i >= st.n || !live(st, i)) return;
     const int s = st.sys[i];
-    if (s < 0 || st.sp[s].nc <= 0) return;
+    if (st.sp[s].nc <= 0) return;
     const int c = nl.cell_of[i];
     const int pos = nl.cell_start[c] + atomicAdd(&nl.cell_fill[c], 1);
     nl.cell_list[pos] = i;
@@ -146,7 +151,7 @@ __global__ void k_nbr_count(nepx_state st, nepx_nlist nl) {
     const int i = blockIdx.x * kNB + threadIdx.x;
     if (i >= st.n) return;
     int nf = 0, nr = 0;
-    if (st.sys[i] >= 0) {
+    if (live(st, i)) {
         const nepx_sysparam& sp = st.sp[st.sys[i]];
         const double r2 = __dmul_rn(sp.r_list, sp.r_list);
         for_candidates(st, nl, i, [&](int j) {
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_rows(nepx_nlist nl, int n) 
 __global__ void k_nbr_fill(nepx_state st, nepx_nlist nl) {
     if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
     const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i >= st.n || st.sys[i] < 0) return;
+    if (i >= st.n || !live(st, i)) return;
     const nepx_sysparam& sp = st.sp[st.sys[i]];
     const double r2 = __dmul_rn(sp.r_list, sp.r_list);
     int pf = nl.row_start[i], pr = nl.rrow_start[i];
@@ -197,37 +202,39 @@ __global__ void k_nbr_fill(nepx_state st, nepx_nlist nl) {
     });
 }
 
-// Sort each row ascending: one block of 256 threads per row, bitonic in LDS.
+// Sort each row ascending: blocks of 256 threads stride over the rows, bitonic in LDS.
+// The grid is capped, so the launch costs little when no rebuild is due.
 __global__ __launch_bounds__(256) void k_sort_rows(nepx_nlist nl, int n, int reverse) {
     if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
     __shared__ int32_t buf[kSortMax];
-    const int i = blockIdx.x;
-    if (i >= n) return;
     const int32_t* start = reverse ? nl.rrow_start : nl.row_start;
     int32_t* data = reverse ? nl.rnbr : nl.nbr;
-    const int b = start[i], len = start[i + 1] - b;
-    if (len <= 1) return;
-    int p2 = 1;
-    while (p2 < len) p2 <<= 1;
-    for (int k = threadIdx.x; k < p2; k += 256) buf[k] = k < len ? data[b + k] : 0x7fffffff;
-    __syncthreads();
-    for (int size = 2; size <= p2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int k = threadIdx.x; k < p2; k += 256) {
-                const int partner = k ^ stride;
-                if (partner > k) {
-                    const bool up = (k & size) == 0;
-                    const int32_t a = buf[k], c = buf[partner];
-                    if ((a > c) == up) {
-                        buf[k] = c;
-                        buf[partner] = a;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int b = start[i], len = start[i + 1] - b;
+        if (len <= 1) continue;  // uniform per block
+        int p2 = 1;
+        while (p2 < len) p2 <<= 1;
+        for (int k = threadIdx.x; k < p2; k += 256) buf[k] = k < len ? data[b + k] : 0x7fffffff;
+        __syncthreads();
+        for (int size = 2; size <= p2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int k = threadIdx.x; k < p2; k += 256) {
+                    const int partner = k ^ stride;
+                    if (partner > k) {
+                        const bool up = (k & size) == 0;
+                        const int32_t a = buf[k], c = buf[partner];
+                        if ((a > c) == up) {
+                            buf[k] = c;
+                            buf[partner] = a;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
+        for (int k = threadIdx.x; k < len; k += 256) data[b + k] = buf[k];
+        __syncthreads();
     }
-    for (int k = threadIdx.x; k < len; k += 256) data[b + k] = buf[k];
 }
 
 // Reverse entries hold the partner i; replace it by the slot of (i, k) in row i.
@@ -317,7 +324,10 @@ __device__ __forceinline__ bool slot_active(const nepx_state& st, const nepx_nli
                                             double& tx, double& ty, double& tz) {
     i = nl.slot_row[s];
     j = nl.nbr[s];
-    const nepx_sysparam& sp = st.sp[st.sys[i]];
+    const int sy = st.sys[i];
+    if (sy < 0 || sy != st.sys[j]) return false;   // slots left over from a freed system
+    const nepx_sysparam& sp = st.sp[sy];
+    if (!sp.active) return false;
     return pair_d2(st, sp.L, i, j, tx, ty, tz) < __dmul_rn(sp.rc, sp.rc);
 }
 
@@ -409,7 +419,7 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
     const int sub = threadIdx.x % kGL;
     if (k >= st.n) return;  // whole 16-lane groups exit together
     double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, u = 0;
-    if (st.sys[k] >= 0 && !nl.flags[F_OVERFLOW]) {
+    if (live(st, k) && !nl.flags[F_OVERFLOW]) {
         for (int s = nl.row_start[k] + sub; s < nl.row_start[k + 1]; s += kGL) {
             const int c = nl.cidx[s];
             if (c < 0) continue;
@@ -496,8 +506,9 @@ extern "C" int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, v
     hipLaunchKernelGGL(k_nbr_count, dim3(gb), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_scan_rows, dim3(2), dim3(kScanBlock), 0, s, *nl, n);
     hipLaunchKernelGGL(k_nbr_fill, dim3(gb), dim3(kNB), 0, s, *st, *nl);
-    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 0);
-    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 1);
+    const int gs = n < 4096 ? n : 4096;
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 0);
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 1);
     hipLaunchKernelGGL(k_rev_slots, dim3(gb), dim3(kNB), 0, s, *nl, n);
     hipLaunchKernelGGL(k_build_done, dim3(gb), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
@@ -535,8 +546,9 @@ extern "C" int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64
     if (m > 0)
         hipLaunchKernelGGL(k_pairs_fill, dim3(blocks_for(m, kNB)), dim3(kNB), 0, s, *nl, pairs, m,
                            nl->cell_fill, nl->cell_fill + n);
-    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 0);
-    hipLaunchKernelGGL(k_sort_rows, dim3(n), dim3(256), 0, s, *nl, n, 1);
+    const int gs = n < 4096 ? n : 4096;
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 0);
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 1);
     hipLaunchKernelGGL(k_rev_slots, dim3(blocks_for(n, kNB)), dim3(kNB), 0, s, *nl, n);
     hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
     NEPX_LAUNCH_CHECK();

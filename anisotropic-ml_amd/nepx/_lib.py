@@ -53,10 +53,10 @@ SYSPARAM_DTYPE = np.dtype([("L", "f8"), ("kT", "f8"), ("dt", "f8"), ("tau", "f8"
                            ("rndof", "f8"), ("moi", "f8", 3), ("r_list", "f8"), ("rc", "f8"),
                            ("skin", "f8"), ("pad", "f8", 4), ("p_start", "i4"), ("p_end", "i4"),
                            ("blk_start", "i4"), ("blk_end", "i4"), ("nc", "i4"), ("cell_off", "i4"),
-                           ("ipad", "i4", 10)])
+                           ("active", "i4"), ("ipad", "i4", 9)])
 THERMO_DTYPE = np.dtype([("xi", "f8"), ("eta", "f8"), ("xi_r", "f8"), ("eta_r", "f8"), ("theta", "f8"),
                          ("e_r", "f8"), ("Kt", "f8"), ("Kr", "f8"), ("U", "f8"), ("max_disp2", "f8"),
-                         ("pad", "f8", 6)])
+                         ("steps", "f8"), ("budget", "f8"), ("pending", "f8"), ("pad", "f8", 3)])
 
 FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW = range(6)
 NVT_STEP_TWO, NVT_TRANSLATE, NVT_ROTATE, NVT_THERMO = 2, 3, 4, 5

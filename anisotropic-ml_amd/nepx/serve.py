@@ -1,0 +1,409 @@
+"""Continuous batching of MD jobs on one GPU.
+
+SlotArena: a fixed-capacity particle arena whose systems come and go.  Every
+capacity (particles, system slots, list slots, cells) is fixed at creation,
+so the hipGraphs captured for the step stay valid while jobs are admitted
+and retired: the kernels read each system's parameters, activity flag and
+step budget from device records.  A job is admitted into a free
+block-aligned particle range when its memory footprint fits; its first
+thermostat update runs on the device for that system only; it advances with
+every other resident system of the same pair model, deactivates itself when
+its step budget is reached (device side, no host round trip) and is
+harvested between chunks of steps.
+
+Server: per pair-model class one SlotArena on its own stream; jobs arrive
+over time (e.g. Poisson), wait in a FIFO queue until they fit, and the
+classes advance concurrently in chunks of hipGraph-replayed steps.  Per-step
+latency comes from events recorded at every step boundary.
+
+run_plain: the same jobs one at a time with the single-system step, for the
+served-versus-plain comparison.
+"""
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import StepRunner
+from .model import EnergyNet
+from .shapes import Shape
+from .system import BLOCK, Arena, SystemSpec, _align
+
+
+@dataclass
+class Job:
+    jid: int
+    cls: str               # pair-model class (shape name)
+    spec: SystemSpec
+    steps: int
+    arrival: float = 0.0   # seconds after the start of the run
+    # filled by the server
+    slot: int = -1
+    admitted_at: float = -1.0
+    done_at: float = -1.0
+    steps_done: int = 0
+    result: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.spec.n
+
+
+class _Intervals:
+    """First-fit allocator over [0, size) in integer units."""
+
+    def __init__(self, size):
+        self.free = [(0, size)]
+
+    def alloc(self, m):
+        for k, (a, b) in enumerate(self.free):
+            if b - a >= m:
+                if b - a == m:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (a + m, b)
+                return a
+        return None
+
+    def release(self, a, m):
+        self.free.append((a, a + m))
+        self.free.sort()
+        merged = []
+        for x, y in self.free:
+            if merged and merged[-1][1] == x:
+                merged[-1] = (merged[-1][0], y)
+            else:
+                merged.append((x, y))
+        self.free = merged
+
+
+class SlotArena(Arena):
+    def __init__(self, shape: Shape, hypers, net: EnergyNet, capacity, max_systems, list_cap=None,
+                 cell_cap=1 << 20, device="cuda", clip=100.0):
+        self._capacity = _align(capacity)
+        self._max_systems = int(max_systems)
+        self._cell_cap = int(cell_cap)
+        self.free_blocks = _Intervals(self._capacity // BLOCK)
+        self.free_cells = _Intervals(self._cell_cap)
+        self.free_slots = list(range(self._max_systems))
+        self.jobs = {}        # slot -> Job
+        self.pair_est = {}    # slot -> estimated list pairs
+        if list_cap is None:
+            list_cap = 200 * self._capacity   # half-list slots per particle at the densest supported packing
+        super().__init__([], shape, hypers, net, device=device, clip=clip, list_cap=list_cap)
+
+    def _layout(self):
+        self.n = self._capacity
+        self.starts = []
+        self.sysrec = np.zeros(self._max_systems, dtype=_lib.SYSPARAM_DTYPE)
+        self.n_cells = max(self._cell_cap, 2 * self.n)
+
+    # ------------------------------------------------------------ admission
+    def footprint(self, spec):
+        r_list = spec.rc + spec.skin
+        rho = spec.n / spec.L ** 3
+        pairs = 0.5 * spec.n * min(spec.n - 1, rho * 4.0 / 3.0 * math.pi * r_list ** 3) * 1.3 + 256
+        return _align(spec.n) // BLOCK, pairs
+
+    def fits_ever(self, job: Job):
+        """Whether the job could be placed in the empty arena (else it is rejected, not queued)."""
+        nblk, pairs = self.footprint(job.spec)
+        r_list = job.spec.rc + job.spec.skin
+        nc = int(math.floor(job.spec.L / r_list))
+        return nblk * BLOCK <= self._capacity and pairs <= 0.9 * self.cap and (nc < 3 or nc ** 3 <= self._cell_cap)
+
+    def admit(self, job: Job):
+        """Place the job if it fits; returns True on success. Call between steps (stream idle)."""
+        spec = job.spec
+        if not self.free_slots:
+            return False
+        nblk, pairs = self.footprint(spec)
+        if sum(self.pair_est.values()) + pairs > 0.9 * self.cap:
+            return False
+        r_list = spec.rc + spec.skin
+        nc = int(math.floor(spec.L / r_list))
+        nc = nc if nc >= 3 else 0
+        b0 = self.free_blocks.alloc(nblk)
+        if b0 is None:
+            return False
+        c0 = self.free_cells.alloc(nc ** 3) if nc else 0
+        if c0 is None:
+            self.free_blocks.release(b0, nblk)
+            return False
+        slot = self.free_slots.pop(0)
+        a = b0 * BLOCK
+        b = a + spec.n
+        dev = self.device
+        for c, key in enumerate("xyz"):
+            self.t[key][a:b].copy_(torch.from_numpy(spec.x[:, c]))
+            self.t["v" + key][a:b].copy_(torch.from_numpy(spec.v[:, c]))
+        for c in range(4):
+            self.t[f"q{c}"][a:b].copy_(torch.from_numpy(spec.q[:, c]))
+            self.t[f"p{c}"][a:b].copy_(torch.from_numpy(spec.p[:, c]))
+        self.t["mass"][a:b].copy_(torch.from_numpy(spec.mass))
+        self.t["sys"][a:b].fill_(slot)
+        r = np.zeros(1, dtype=_lib.SYSPARAM_DTYPE)[0]
+        r["L"], r["kT"], r["dt"], r["tau"] = spec.L, spec.kT, spec.dt, spec.tau
+        r["ndof"], r["rndof"] = spec.ndof, spec.rndof
+        r["moi"] = spec.moi
+        r["r_list"], r["rc"], r["skin"] = r_list, spec.rc, spec.skin
+        r["p_start"], r["p_end"] = a, b
+        r["blk_start"], r["blk_end"] = b0, b0 + nblk
+        r["nc"], r["cell_off"] = nc, c0
+        r["active"] = 1
+        th = np.zeros(1, dtype=_lib.THERMO_DTYPE)[0]
+        for key, val in spec.thermo.items():
+            th[key] = val
+        th["theta"] = th["e_r"] = 1.0
+        th["budget"] = float(job.steps)
+        th["pending"] = 1.0
+        self._put_record("sp", slot, r)
+        self._put_record("th", slot, th)
+        self.jobs[slot] = job
+        self.pair_est[slot] = pairs
+        self.blk_of = getattr(self, "blk_of", {})
+        self.blk_of[slot] = (b0, nblk, c0, nc ** 3)
+        job.slot = slot
+        return True
+
+    def _put_record(self, key, slot, rec):
+        raw = np.frombuffer(rec.tobytes(), dtype=np.uint8)
+        sz = len(raw)
+        self.t[key][slot * sz:(slot + 1) * sz].copy_(torch.from_numpy(raw.copy()))
+
+    def commit_admissions(self, stream=None):
+        """Initial kinetic energies + first thermostat update of the newly admitted systems, then a
+        forced list rebuild at the next step."""
+        _lib.check(_lib.lib.nepx_nvt_init(C_byref(self.st), C_byref(self.nl), 2, self._s(stream)), "nepx_nvt_init")
+        self.lt["flags"][_lib.FLAG_REBUILD:_lib.FLAG_REBUILD + 1].fill_(1)
+
+    # ------------------------------------------------------------ retirement
+    def records(self):
+        sp = self.t["sp"].cpu().numpy().view(_lib.SYSPARAM_DTYPE)
+        th = self.t["th"].cpu().numpy().view(_lib.THERMO_DTYPE)
+        return sp, th
+
+    def harvest(self, keep_state=True):
+        """Retire systems that finished their budget; returns their jobs (with results)."""
+        if not self.jobs:
+            return []
+        sp, th = self.records()
+        done = []
+        for slot, job in list(self.jobs.items()):
+            job.steps_done = int(th[slot]["steps"])
+            if sp[slot]["active"] == 0:
+                a, b = int(sp[slot]["p_start"]), int(sp[slot]["p_end"])
+                if keep_state:
+                    g = lambda key: self.t[key][a:b].cpu().numpy()
+                    job.result = dict(x=np.stack([g("x"), g("y"), g("z")], 1),
+                                      v=np.stack([g("vx"), g("vy"), g("vz")], 1),
+                                      q=np.stack([g("q0"), g("q1"), g("q2"), g("q3")], 1),
+                                      p=np.stack([g("p0"), g("p1"), g("p2"), g("p3")], 1),
+                                      thermo={k: float(th[slot][k]) for k in ("xi", "eta", "xi_r", "eta_r",
+                                                                              "Kt", "Kr", "U", "steps")})
+                self.t["sys"][a:b].fill_(-1)
+                self._put_record("sp", slot, np.zeros(1, dtype=_lib.SYSPARAM_DTYPE)[0])
+                b0, nblk, c0, ncell = self.blk_of.pop(slot)
+                self.free_blocks.release(b0, nblk)
+                if ncell:
+                    self.free_cells.release(c0, ncell)
+                self.free_slots.append(slot)
+                self.free_slots.sort()
+                del self.jobs[slot]
+                del self.pair_est[slot]
+                done.append(job)
+        if done:
+            self.lt["flags"][_lib.FLAG_REBUILD:_lib.FLAG_REBUILD + 1].fill_(1)
+        return done
+
+    def particle_steps(self):
+        """Particle-steps done so far by the resident systems."""
+        if not self.jobs:
+            return 0
+        _, th = self.records()
+        return sum(job.n * int(th[slot]["steps"]) for slot, job in self.jobs.items())
+
+
+def C_byref(x):
+    import ctypes
+    return ctypes.byref(x)
+
+
+class _ClassRunner:
+    def __init__(self, arena: SlotArena):
+        self.arena = arena
+        self.runner = StepRunner(arena, use_graph=True)
+        self.runner.stream.synchronize()
+        with torch.cuda.stream(self.runner.stream):
+            arena.build_list(True, self.runner.stream)   # an empty list; real builds follow admissions
+        self.runner.stream.synchronize()
+        self.runner._capture()
+        self.step_times = []
+
+    def run_chunk(self, k):
+        st = self.runner.stream
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
+        evs[0].record(st)
+        for i in range(k):
+            self.runner.step()
+            evs[i + 1].record(st)
+        return evs
+
+
+class Server:
+    """Continuous batching: one SlotArena per pair-model class, classes on separate streams."""
+
+    def __init__(self, classes, capacity, max_systems, chunk=20, device="cuda"):
+        # classes: name -> (Shape, hypers, EnergyNet)
+        self.device = torch.device(device)
+        self.classes = {}
+        for name, (shape, hyp, net) in classes.items():
+            ar = SlotArena(shape, hyp, net, capacity, max_systems, device=self.device)
+            self.classes[name] = _ClassRunner(ar)
+        self.chunk = chunk
+
+    def run(self, jobs, time_limit=None, log=None):
+        queue = collections.deque(sorted(jobs, key=lambda j: j.arrival))
+        waiting = {name: collections.deque() for name in self.classes}
+        finished = []
+        rejected = []
+        lat = []
+        t0 = time.perf_counter()
+        retired_ps = 0
+        last_log = 0.0
+        while True:
+            now = time.perf_counter() - t0
+            if time_limit is not None and now >= time_limit:
+                break
+            while queue and queue[0].arrival <= now:
+                j = queue.popleft()
+                waiting[j.cls].append(j)
+            # admission (FIFO per class; a job that does not fit blocks the ones behind it); all device
+            # work of a class, copies included, stays on that class's stream
+            for name, cr in self.classes.items():
+                while waiting[name] and not cr.arena.fits_ever(waiting[name][0]):
+                    j = waiting[name].popleft()
+                    j.result = {"error": "job exceeds the arena capacity"}
+                    rejected.append(j)
+                with torch.cuda.stream(cr.runner.stream):
+                    admitted = False
+                    while waiting[name] and cr.arena.admit(waiting[name][0]):
+                        j = waiting[name].popleft()
+                        j.admitted_at = now
+                        admitted = True
+                    if admitted:
+                        cr.arena.commit_admissions(cr.runner.stream)
+            busy = [cr for cr in self.classes.values() if cr.arena.jobs]
+            if not busy:
+                if not queue and not any(waiting.values()):
+                    break
+                time.sleep(0.001)
+                continue
+            evs = [(cr, cr.run_chunk(self.chunk)) for cr in busy]
+            for cr, ev in evs:
+                cr.runner.stream.synchronize()
+                lat.extend(ev[i].elapsed_time(ev[i + 1]) for i in range(len(ev) - 1))
+                with torch.cuda.stream(cr.runner.stream):
+                    retired = cr.arena.harvest()
+                for j in retired:
+                    j.done_at = time.perf_counter() - t0
+                    retired_ps += j.n * j.steps
+                    finished.append(j)
+            if log and now - last_log > 5.0:
+                last_log = now
+                log(now, {n: len(c.arena.jobs) for n, c in self.classes.items()}, len(finished))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        resident_ps = sum(cr.arena.particle_steps() for cr in self.classes.values())
+        for cr in self.classes.values():
+            cr.arena.check()
+        self.last_finished = finished
+        self.last_rejected = rejected
+        st = _stats("served", finished, retired_ps + resident_ps, wall, lat)
+        st["jobs_rejected"] = len(rejected)
+        return st
+
+
+def _stats(mode, finished, particle_steps, wall, lat):
+    lat = np.asarray(lat) if len(lat) else np.zeros(1)
+    return {"mode": mode, "particle_steps": int(particle_steps), "wall_s": wall,
+            "particle_steps_per_s": particle_steps / wall, "jobs_finished": len(finished),
+            "step_ms_p50": float(np.percentile(lat, 50)), "step_ms_p99": float(np.percentile(lat, 99)),
+            "steps_timed": int(lat.size)}
+
+
+def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None):
+    """The same jobs one at a time, each with its own single-system arena and captured step."""
+    dev = torch.device(device)
+    t0 = time.perf_counter()
+    last_log = 0.0
+    done_ps = 0
+    finished = []
+    lat = []
+    for j in sorted(jobs, key=lambda j: j.arrival):
+        now = time.perf_counter() - t0
+        if time_limit is not None and now >= time_limit:
+            break
+        if j.arrival > now:
+            time.sleep(j.arrival - now)
+        shape, hyp, net = classes[j.cls]
+        ar = Arena([j.spec], shape, hyp, net, device=dev)
+        runner = StepRunner(ar, use_graph=True)
+        runner.init()
+        left = j.steps
+        while left > 0:
+            if time_limit is not None and time.perf_counter() - t0 >= time_limit:
+                break
+            k = min(chunk, left)
+            st = runner.stream
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
+            evs[0].record(st)
+            for i in range(k):
+                runner.step()
+                evs[i + 1].record(st)
+            st.synchronize()
+            lat.extend(evs[i].elapsed_time(evs[i + 1]) for i in range(k))
+            left -= k
+            done_ps += j.n * k
+            if log and time.perf_counter() - t0 - last_log > 5.0:
+                last_log = time.perf_counter() - t0
+                log(last_log, {"job": j.jid, "n": j.n, "left": left}, len(finished))
+        j.steps_done = j.steps - left
+        if left == 0:
+            j.done_at = time.perf_counter() - t0
+            a = ar.system_arrays(0)
+            j.result = dict(x=a["x"], v=a["v"], q=a["q"], p=a["p"],
+                            thermo={k: float(ar.thermo()[0][k]) for k in ("xi", "eta", "xi_r", "eta_r", "Kt", "Kr",
+                                                                          "U", "steps")})
+            finished.append(j)
+        ar.check()
+    torch.cuda.synchronize()
+    return _stats("plain", finished, done_ps, time.perf_counter() - t0, lat)
+
+
+def poisson_jobs(rate, duration, seed, shapes=("cube8", "tet10"), n_range=(128, 65536),
+                 steps_range=(1000, 100000), kT_range=(0.5, 2.0), phi_range=(0.2, 0.45), max_jobs=None):
+    """Seeded job trace: Poisson arrivals, N and step budget log-uniform, kT and packing uniform."""
+    from .configs import make_system
+    from .shapes import get_shape
+    rng = np.random.default_rng(seed)
+    t, jobs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration or (max_jobs is not None and len(jobs) >= max_jobs):
+            break
+        shape = shapes[rng.integers(len(shapes))]
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        steps = int(round(math.exp(rng.uniform(math.log(steps_range[0]), math.log(steps_range[1])))))
+        kT = float(rng.uniform(*kT_range))
+        phi = float(rng.uniform(*phi_range))
+        sh = get_shape(shape)
+        # the minimum image needs a box of at least 2 rc: small systems get a lower packing fraction
+        phi = min(phi, n * sh.volume / (2.05 * sh.cutoff) ** 3)
+        spec = make_system(shape, n, phi, kT=kT, seed=int(rng.integers(1 << 30)), non_overlap=False)
+        jobs.append(Job(len(jobs), shape, spec, steps, t))
+    return jobs

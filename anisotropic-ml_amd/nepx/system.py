@@ -99,6 +99,7 @@ class Arena:
             r["p_start"], r["p_end"] = starts[k], starts[k] + sp.n
             r["blk_start"], r["blk_end"] = starts[k] // BLOCK, (starts[k] + sp.n + BLOCK - 1) // BLOCK
             r["nc"], r["cell_off"] = nc, cell_off
+            r["active"] = 1
             cell_off += nc ** 3
         self.sysrec = rec
         self.n_cells = max(cell_off, 2 * self.n)
@@ -130,7 +131,7 @@ class Arena:
         self.t["mass"].copy_(torch.from_numpy(mass))
         self.t["sys"] = torch.from_numpy(sysid).to(dev)
         self.t["sp"] = torch.from_numpy(self.sysrec.view(np.uint8).copy()).to(dev)
-        th = np.zeros(len(self.specs), dtype=_lib.THERMO_DTYPE)
+        th = np.zeros(len(self.sysrec), dtype=_lib.THERMO_DTYPE)
         for k, sp in enumerate(self.specs):
             for key, val in sp.thermo.items():
                 th[k][key] = val
@@ -140,7 +141,7 @@ class Arena:
         self.t["partials"] = torch.zeros(4 * (n // BLOCK + 1), dtype=torch.float64, device=dev)
         self.t["dp"] = torch.zeros(4 * n, dtype=torch.float64, device=dev)
         st = _lib.State()
-        st.n, st.n_sys = n, len(self.specs)
+        st.n, st.n_sys = n, len(self.sysrec)
         for k in _lib._STATE_PTRS:
             setattr(st, k, self.t[k].data_ptr())
         self.st = st

@@ -76,6 +76,40 @@ def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
                       f"update timed on the full state"}
 
 
+def serve_bench(a):
+    """Served (continuous batching) vs plain (one system at a time) on the same seeded job trace."""
+    import copy
+    import torch
+    import nepx
+    from nepx.serve import Server, poisson_jobs, run_plain
+    dev = torch.device("cuda:0")
+    net = nepx.EnergyNet.random(40, [64, 64], seed=7, device=dev)
+    classes = {name: (nepx.get_shape(name), (11, 6, 4), net) for name in ("cube8", "tet10")}
+    jobs = poisson_jobs(rate=20.0, duration=a.serve_arrivals, seed=2025, shapes=("cube8", "tet10"))
+    plain_jobs = copy.deepcopy(jobs)
+    def log(t, resident, done):
+        print(f"[serve] t={t:.0f}s resident={resident} finished={done}", file=sys.stderr, flush=True)
+    srv = Server(classes, capacity=a.serve_capacity, max_systems=1024, chunk=a.serve_chunk, device=dev)
+    served = srv.run(jobs, time_limit=a.serve_seconds, log=log)
+    plain = run_plain(plain_jobs, classes, time_limit=a.serve_seconds, chunk=a.serve_chunk, device=dev, log=log)
+    return {
+        "metric": "served-batch vs plain particle-steps/s (mixed cube/tet job trace); p99 step latency",
+        "value": served["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "served_over_plain": served["particle_steps_per_s"] / plain["particle_steps_per_s"],
+        "served": served,
+        "plain": plain,
+        "config": {"workload": f"Poisson 20 jobs/s for {a.serve_arrivals} s of arrivals ({len(jobs)} jobs), "
+                               "cube8/tet10, N log-uniform [128, 65536], steps log-uniform [1e3, 1e5], "
+                               "kT U[0.5, 2], phi U[0.2, 0.45], FFNN 40->64->64->1 tanh",
+                   "window_s": a.serve_seconds, "capacity_per_class": a.serve_capacity, "chunk": a.serve_chunk},
+        "dtype": "f32 pair math (MFMA f32), f64 integrator",
+        "data": "synthetic",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +121,19 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="after the timed region, copy MIB MiB device-to-device once (a kernel of known traffic "
                          "for scaling profiler byte counters)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve workload: measurement window per mode")
+    ap.add_argument("--serve-arrivals", type=float, default=20.0, help="serve workload: seconds of Poisson arrivals")
+    ap.add_argument("--serve-capacity", type=int, default=262144, help="serve workload: particles per class arena")
+    ap.add_argument("--serve-chunk", type=int, default=20, help="serve workload: steps per scheduling chunk")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "nepx runs on one GPU per process; multi-GPU runs are not part of this build "
                                    "(independent systems shard by placing whole systems per GPU)"}))
         sys.exit(2)
+
+    if a.workload == "serve":
+        print(json.dumps(serve_bench(a)))
+        return
 
     import numpy as np
     import torch

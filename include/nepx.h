@@ -57,7 +57,8 @@ typedef struct nepx_sysparam {
     int32_t p_start, p_end;              /* particle range (block aligned start) */
     int32_t blk_start, blk_end;          /* integrator block range               */
     int32_t nc, cell_off;                /* cells per side (0 = all pairs), offset */
-    int32_t ipad[10];
+    int32_t active;                      /* 0: the system is skipped by every kernel */
+    int32_t ipad[9];
 } nepx_sysparam;
 
 /* Thermostat record per system (Sim.py:99-111, IntegrateStepTwoKernels.py:74-130). */
@@ -66,7 +67,9 @@ typedef struct nepx_thermo {
     double theta, e_r;               /* exp(-xi dt/2), exp(-xi_r dt/2)            */
     double Kt, Kr, U;                /* kinetic (trans, rot) and potential energy */
     double max_disp2;                /* largest squared displacement since build */
-    double pad[6];
+    double steps, budget;            /* steps done; the system deactivates itself at budget (> 0) */
+    double pending;                  /* != 0: initial kinetic energies + first update still to do */
+    double pad[3];
 } nepx_thermo;
 
 /* Particle arena, SoA, device pointers (Sim.py:147-169 holds the same state AoS f32). */
@@ -189,7 +192,9 @@ int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* pair_out, vo
 int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream);
 /* Kinetic energies of the current state and, when update_thermostat != 0, the
  * thermostat update that the reference performs at the start of the first step
- * (Sim.py:147-169).  A run resumed from a saved thermostat state passes 0. */
+ * (Sim.py:147-169).  A run resumed from a saved thermostat state passes 0.
+ * update_thermostat == 2 touches only systems whose record has pending != 0 (newly
+ * admitted systems joining running ones) and clears the flag. */
 int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_thermostat, void* stream);
 /* One part of the step, for callers that drive the reference's methods one by one:
  * NEPX_NVT_STEP_TWO  = integrate_step_two minus the thermostat update (NVT.py:17-64)

@@ -161,3 +161,16 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_interval_allocator_first_fit_and_merge():
+    from nepx.serve import _Intervals
+    a = _Intervals(10)
+    x, y, z = a.alloc(3), a.alloc(4), a.alloc(3)
+    assert (x, y, z) == (0, 3, 7) and a.alloc(1) is None
+    a.release(3, 4)
+    assert a.alloc(5) is None and a.alloc(4) == 3
+    a.release(0, 3)
+    a.release(3, 4)
+    a.release(7, 3)
+    assert a.free == [(0, 10)]
