@@ -40,6 +40,28 @@ struct NetLayout {
     __host__ __device__ constexpr int total() const { return bout_off() + 4; }
 };
 
+__host__ __device__ constexpr int round32(int v) { return (v + 31) / 32 * 32; }
+
+// Offsets (in halves) of the fp16 hi/lo operands of the split-precision path.
+// Each (M tile, 32-deep k-step) holds 64 lanes x 8 halves; the lo part of a
+// matrix follows its hi part.  Input rows are padded to a multiple of 32.
+struct NetLayoutH {
+    int DPH, DP, HP, NL;
+    __host__ __device__ constexpr int fwd_size(int k) const {  // hi + lo
+        return 2 * (HP / 16) * ((k == 0 ? DPH : HP) / 32) * 512;
+    }
+    __host__ __device__ constexpr int bwd_size(int k) const {
+        return 2 * ((k == 0 ? DP : HP) / 16) * (HP / 32) * 512;
+    }
+    __host__ __device__ constexpr int fwd_off(int k) const {
+        return k == 0 ? 0 : fwd_off(k - 1) + fwd_size(k - 1);
+    }
+    __host__ __device__ constexpr int bwd_off(int k) const {
+        return k == 0 ? fwd_off(NL) : bwd_off(k - 1) + bwd_size(k - 1);
+    }
+    __host__ __device__ constexpr int total() const { return bwd_off(NL); }
+};
+
 template <int DP_, int HP_, int NL_>
 struct NetCfg {
     static constexpr int DP = DP_, HP = HP_, NL = NL_;
@@ -47,7 +69,12 @@ struct NetCfg {
     static constexpr int KD = DP / 4;    // k-steps over the input
     static constexpr int KH = HP / 4;    // k-steps over a hidden layer
     static constexpr int MD = DP / 16;   // M tiles of dE/dG
+    static constexpr int DPH = round32(DP);
+    static constexpr int MDH = DPH / 16; // input chunks of the fp16 path
+    static constexpr bool HALF_OK = (HP % 32) == 0;
+    static_assert(MD <= MT, "dE/dG tiles reuse the hidden-layer accumulators");
     static constexpr NetLayout layout() { return NetLayout{DP, HP, NL}; }
+    static constexpr NetLayoutH layout_h() { return NetLayoutH{DPH, DP, HP, NL}; }
 };
 
 __device__ __forceinline__ float act_fwd(float z, int act) {
@@ -175,13 +202,158 @@ __device__ __forceinline__ void ffnn_tiles(const float* __restrict__ W, int act,
     }
 }
 
-// Copy the packed network into LDS (all threads of the block cooperate).
+// ---------------------------------------------------------------------------
+// Split-precision network: every product x*w is formed as
+// x_hi w_hi + x_hi w_lo + x_lo w_hi with fp16 parts (x = x_hi + x_lo exactly
+// to ~22 bits) on v_mfma_f32_16x16x32_f16, FP32 accumulation.  K-step s of 32
+// takes, in lane (p, g), units 16*(2s + j/4) + 4g + j%4 for j = 0..7: the two
+// 16-unit accumulator tiles 2s, 2s+1 of the previous layer, so there is still
+// no cross-lane movement.  Deltas are scaled by 2^10 before the split (exact)
+// to stay clear of fp16 underflow, and the products scaled back.
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split8(const v4f& a, const v4f& b, float sc, v8h& hi, v8h& lo) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float x = (j < 4 ? a[j] : b[j - 4]) * sc;
+        const _Float16 h = (_Float16)x;
+        hi[j] = h;
+        lo[j] = (_Float16)(x - (float)h);
+    }
+}
+
+__device__ __forceinline__ v4f mfma3(const v8h* __restrict__ A, int idx, int lo_off, const v8h& bh, const v8h& bl,
+                                     v4f acc) {
+    const v8h ah = A[idx], al = A[idx + lo_off];
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+    return acc;
+}
+
+// Same contract as ffnn_tiles; gin has MDH chunks (zero past D).  W32 supplies
+// biases and the output layer, WH the fp16 operands (global or LDS).
+template <class N, int NT>
+__device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, const _Float16* __restrict__ WH, int act,
+                                             const v4f (&gin)[NT][N::MDH], float scale, v4f (&dgo)[NT][N::MD],
+                                             float (&y)[NT]) {
+    constexpr NetLayout Lay = N::layout();
+    constexpr NetLayoutH LH = N::layout_h();
+    constexpr float kUp = 1024.f, kDown = 1.f / 1024.f;
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int g = lane >> 4;
+    const v8h* A8 = reinterpret_cast<const v8h*>(WH);
+    v4f h[N::NL][NT][N::MT];
+#pragma unroll
+    for (int k = 0; k < N::NL; ++k) {
+        const int base = LH.fwd_off(k) / 8;
+        const int ks = (k == 0 ? N::DPH : N::HP) / 32;
+        const int lo_off = (N::MT * ks * 512) / 8;
+        const float* bk = W32 + Lay.bias_off(k);
+        v4f acc[NT][N::MT];
+#pragma unroll
+        for (int mt = 0; mt < N::MT; ++mt) {
+            const float4 b4 = *reinterpret_cast<const float4*>(bk + 16 * mt + 4 * g);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t][mt] = v4f{b4.x, b4.y, b4.z, b4.w};
+        }
+#pragma unroll
+        for (int s = 0; s < ks; ++s) {
+            v8h bh[NT], bl[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                if (k == 0) split8(gin[t][2 * s], gin[t][2 * s + 1], 1.f, bh[t], bl[t]);
+                else split8(h[k - 1][t][2 * s], h[k - 1][t][2 * s + 1], 1.f, bh[t], bl[t]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt) {
+                const int idx = base + (mt * ks + s) * 64 + lane;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t][mt] = mfma3(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) h[k][t][mt][r] = act_fwd(acc[t][mt][r], act);
+    }
+    const float* wo = W32 + Lay.wout_off();
+    v4f delta[NT][N::MT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) y[t] = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < N::MT; ++mt) {
+        const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * mt + 4 * g);
+        const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                y[t] += wv[r] * h[N::NL - 1][t][mt][r];
+                delta[t][mt][r] = scale * wv[r] * act_der(h[N::NL - 1][t][mt][r], act);
+            }
+    }
+    const float bo = W32[Lay.bout_off()];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        y[t] += __shfl_xor(y[t], 16);
+        y[t] += __shfl_xor(y[t], 32);
+        y[t] += bo;
+    }
+#pragma unroll
+    for (int k = N::NL - 1; k >= 0; --k) {
+        const int base = LH.bwd_off(k) / 8;
+        constexpr int ks = N::HP / 32;
+        const int mtn = (k == 0 ? N::MD : N::MT);
+        const int lo_off = (mtn * ks * 512) / 8;
+        v4f acc[NT][N::MT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt) acc[t][mt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < ks; ++s) {
+            v8h bh[NT], bl[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) split8(delta[t][2 * s], delta[t][2 * s + 1], kUp, bh[t], bl[t]);
+#pragma unroll
+            for (int mt = 0; mt < N::MT; ++mt) {
+                if (mt >= mtn) break;
+                const int idx = base + (mt * ks + s) * 64 + lane;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[t][mt] = mfma3(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
+            }
+        }
+        if (k > 0) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int mt = 0; mt < N::MT; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        delta[t][mt][r] = acc[t][mt][r] * kDown * act_der(h[k - 1][t][mt][r], act);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+#pragma unroll
+                for (int mt = 0; mt < N::MD; ++mt) dgo[t][mt] = acc[t][mt] * kDown;
+        }
+    }
+}
+
+// Copy `floats` floats (a multiple of 4) into LDS (all threads of the block cooperate).
+__device__ __forceinline__ void stage_copy(float* __restrict__ lds, const float* __restrict__ src, int floats) {
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(lds);
+    for (int k = threadIdx.x; k < floats / 4; k += blockDim.x) d4[k] = s4[k];
+}
+
 template <class N>
 __device__ __forceinline__ void stage_weights(float* __restrict__ lds, const float* __restrict__ W) {
-    constexpr int total = N::layout().total();
-    const float4* src = reinterpret_cast<const float4*>(W);
-    float4* dst = reinterpret_cast<float4*>(lds);
-    for (int k = threadIdx.x; k < total / 4; k += blockDim.x) dst[k] = src[k];
+    stage_copy(lds, W, N::layout().total());
     __syncthreads();
 }
 

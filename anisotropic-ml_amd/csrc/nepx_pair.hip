@@ -116,12 +116,11 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_mode
 
 // Input chunks of one pair for the MFMA tile: chunk c holds inputs
 // 16c + 4*(lane/16) + 0..3 (zero past D).
-template <class N>
-__device__ __forceinline__ void gather_chunks(const float* __restrict__ row, bool valid, int D,
-                                              v4f (&gin)[N::MD]) {
+template <int MC>
+__device__ __forceinline__ void gather_chunks(const float* __restrict__ row, bool valid, int D, v4f (&gin)[MC]) {
     const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
-    for (int c = 0; c < N::MD; ++c) {
+    for (int c = 0; c < MC; ++c) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int k = 16 * c + 4 * g + r;
@@ -130,12 +129,11 @@ __device__ __forceinline__ void gather_chunks(const float* __restrict__ row, boo
     }
 }
 
-template <class N>
-__device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool valid, int D,
-                                               const v4f (&v)[N::MD]) {
+template <int MC>
+__device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool valid, int D, const v4f (&v)[MC]) {
     const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
-    for (int c = 0; c < N::MD; ++c) {
+    for (int c = 0; c < MC; ++c) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int k = 16 * c + 4 * g + r;
@@ -144,50 +142,82 @@ __device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool val
     }
 }
 
-// Networks whose packed weights fit this budget are staged in LDS once per block.
+// Networks whose packed operands fit this budget are staged in LDS once per block.
 constexpr int kWeightLdsBytes = 64 * 1024;
-template <class N>
-constexpr bool weights_in_lds() {
-    return N::layout().total() * 4 <= kWeightLdsBytes;
+template <class N, bool HALF>
+constexpr int operand_floats() {
+    return HALF ? N::layout_h().total() / 2 : N::layout().total();
 }
-template <class N>
+template <class N, bool HALF>
+constexpr bool weights_in_lds() {
+    return operand_floats<N, HALF>() * 4 <= kWeightLdsBytes;
+}
+template <class N, bool HALF>
 constexpr int weight_lds_floats() {
-    return weights_in_lds<N>() ? N::layout().total() : 4;
+    return weights_in_lds<N, HALF>() ? operand_floats<N, HALF>() : 4;
 }
 
 constexpr int kNetTiles = 2;  // 16-pair tiles sharing each A-operand read
 
-template <class N, int NT>
-__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, int act, float en_min,
-                                                 float scale, int D, const float* __restrict__ G,
+// Network over NT tiles of 16 pairs with the operands chosen at compile time.
+template <class N, int NT, bool HALF>
+struct NetRun {
+    static constexpr int IC = HALF ? N::MDH : N::MD;  // input chunks
+    __device__ __forceinline__ static void run(const float* W, const _Float16* WH, int act, const v4f (&gin)[NT][IC],
+                                               float scale, v4f (&dg)[NT][N::MD], float (&y)[NT]) {
+        if constexpr (HALF) ffnn_tiles_h<N, NT>(W, WH, act, gin, scale, dg, y);
+        else ffnn_tiles<N, NT>(W, act, gin, scale, dg, y);
+    }
+};
+
+// Stage the operands of this precision into LDS when they fit; returns the pointers to use.
+template <class N, bool HALF>
+__device__ __forceinline__ void stage_net(float* wl, const float* Wg, const void* Whg, const float*& W,
+                                          const _Float16*& WH) {
+    W = Wg;
+    WH = reinterpret_cast<const _Float16*>(Whg);
+    if constexpr (weights_in_lds<N, HALF>()) {
+        if constexpr (HALF) {
+            stage_copy(wl, reinterpret_cast<const float*>(Whg), operand_floats<N, HALF>());
+            WH = reinterpret_cast<const _Float16*>(wl);
+        } else {
+            stage_copy(wl, Wg, operand_floats<N, HALF>());
+            W = wl;
+        }
+        __syncthreads();
+    }
+}
+
+template <class N, int NT, bool HALF>
+__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, const void* __restrict__ Whg, int act,
+                                                 float en_min, float scale, int D, const float* __restrict__ G,
                                                  const int32_t* __restrict__ np_dev, int64_t maxp,
                                                  float* __restrict__ E, float* __restrict__ dEdG) {
-    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N>()];
-    const float* W = Wg;
-    if constexpr (weights_in_lds<N>()) {
-        stage_weights<N>(wl, Wg);
-        W = wl;
-    }
+    using R = NetRun<N, NT, HALF>;
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, HALF>()];
+    const float* W;
+    const _Float16* WH;
+    stage_net<N, HALF>(wl, Wg, Whg, W, WH);
     const int P = n_pairs(np_dev, maxp);
     const int lane = threadIdx.x & 63;
     const int nt = (P + 16 * NT - 1) / (16 * NT);
     const int wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     const int nwaves = (gridDim.x * kBlock) >> 6;
     for (int tile = wave; tile < nt; tile += nwaves) {
-        v4f gin[NT][N::MD];
+        v4f gin[NT][R::IC];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int p = (tile * NT + t) * 16 + (lane & 15);
-            gather_chunks<N>(G + (size_t)p * D, p < P, D, gin[t]);
+            gather_chunks<R::IC>(G + (size_t)p * D, p < P, D, gin[t]);
         }
         v4f dg[NT][N::MD];
         float y[NT];
-        ffnn_tiles<N, NT>(W, act, gin, scale, dg, y);
+        R::run(W, WH, act, gin, scale, dg, y);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int p = (tile * NT + t) * 16 + (lane & 15);
             if (p < P && lane < 16) E[p] = en_min + scale * y[t];
-            scatter_chunks<N>(dEdG + (size_t)p * D, p < P, D, dg[t]);
+            scatter_chunks<N::MD>(dEdG + (size_t)p * D, p < P, D, dg[t]);
         }
     }
 }
@@ -197,8 +227,9 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, i
 // block runs the same number of iterations and the barriers are uniform.
 // The forward accumulators A of each lane's pair stay in registers through
 // the network phase and feed the adjoint directly.
-template <class Cfg, class N>
+template <class Cfg, class N, bool HALF>
 __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model m, const float* __restrict__ Wg,
+                                                                const void* __restrict__ Whg,
                                                                 int act, float en_min, float scale,
                                                                 const float4* __restrict__ qf,
                                                                 const int32_t* __restrict__ pi,
@@ -209,12 +240,11 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
     constexpr int LD = (Cfg::D + 3) / 4 * 4 + 4;  // row stride in LDS (16-byte aligned)
     __shared__ __attribute__((aligned(16))) float tile[kBlock * LD];
     __shared__ float etile[kBlock];
-    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N>()];
-    const float* W = Wg;
-    if constexpr (weights_in_lds<N>()) {
-        stage_weights<N>(wl, Wg);
-        W = wl;
-    }
+    using R = NetRun<N, kNetTiles, HALF>;
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, HALF>()];
+    const float* W;
+    const _Float16* WH;
+    stage_net<N, HALF>(wl, Wg, Whg, W, WH);
     const int P = n_pairs(np_dev, maxp);
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -237,20 +267,20 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
         __syncthreads();
 #pragma unroll 1
         for (int sub = 0; sub < 64 / (16 * kNetTiles); ++sub) {
-            v4f gin[kNetTiles][N::MD];
+            v4f gin[kNetTiles][R::IC];
 #pragma unroll
             for (int tt = 0; tt < kNetTiles; ++tt) {
                 const int local = wv * 64 + (sub * kNetTiles + tt) * 16 + (lane & 15);
-                gather_chunks<N>(tile + local * LD, true, Cfg::D, gin[tt]);
+                gather_chunks<R::IC>(tile + local * LD, true, Cfg::D, gin[tt]);
             }
             v4f dg[kNetTiles][N::MD];
             float y[kNetTiles];
-            ffnn_tiles<N, kNetTiles>(W, act, gin, scale, dg, y);
+            R::run(W, WH, act, gin, scale, dg, y);
 #pragma unroll
             for (int tt = 0; tt < kNetTiles; ++tt) {
                 const int local = wv * 64 + (sub * kNetTiles + tt) * 16 + (lane & 15);
                 if (lane < 16) etile[local] = en_min + scale * y[tt];
-                scatter_chunks<N>(tile + local * LD, true, Cfg::D, dg[tt]);
+                scatter_chunks<N::MD>(tile + local * LD, true, Cfg::D, dg[tt]);
             }
         }
         __syncthreads();
@@ -436,6 +466,56 @@ extern "C" int nepx_net_pack(int32_t d_in, int32_t width, int32_t n_hidden, cons
     return NEPX_OK;
 }
 
+extern "C" int64_t nepx_net_packed_halves(int32_t d_in, int32_t width, int32_t n_hidden) {
+    if (d_in <= 0 || width <= 0 || n_hidden < 0 || round16(width) % 32) return -1;
+    NetLayoutH L{round32(round16(d_in)), round16(d_in), round16(width), n_hidden + 1};
+    return L.total();
+}
+
+extern "C" int nepx_net_pack_half(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
+                                  uint16_t* out) {
+    NEPX_REQUIRE(d_in > 0 && width > 0 && n_hidden >= 0, NEPX_EINVAL, "bad network dims");
+    NEPX_REQUIRE(round16(width) % 32 == 0, NEPX_EUNSUP, "fp16 path needs a padded width that is a multiple of 32");
+    NEPX_REQUIRE(ws && out, NEPX_EINVAL, "null pointer");
+    const int DP = round16(d_in), DPH = round32(DP), HP = round16(width), NL = n_hidden + 1;
+    NetLayoutH L{DPH, DP, HP, NL};
+    _Float16* o = reinterpret_cast<_Float16*>(out);
+    for (int64_t k = 0; k < L.total(); ++k) o[k] = (_Float16)0.f;
+    auto put = [&](int base, int lo_off, int idx, float w) {
+        const _Float16 h = (_Float16)w;
+        o[base + idx] = h;
+        o[base + lo_off + idx] = (_Float16)(w - (float)h);
+    };
+    for (int k = 0; k < NL; ++k) {
+        const int kin = k == 0 ? d_in : width;
+        const float* W = ws[k];
+        NEPX_REQUIRE(W, NEPX_EINVAL, "null layer %d", k);
+        const int ksf = (k == 0 ? DPH : HP) / 32;
+        const int lof = (HP / 16) * ksf * 512;
+        for (int mt = 0; mt < HP / 16; ++mt)
+            for (int s = 0; s < ksf; ++s)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int j = 0; j < 8; ++j) {
+                        const int row = 16 * mt + lane % 16;
+                        const int col = 16 * (2 * s + j / 4) + 4 * (lane / 16) + j % 4;
+                        const float w = (row < width && col < kin) ? W[row * kin + col] : 0.f;
+                        put(L.fwd_off(k), lof, ((mt * ksf + s) * 64 + lane) * 8 + j, w);
+                    }
+        const int mtb = (k == 0 ? DP : HP) / 16, ksb = HP / 32;
+        const int lob = mtb * ksb * 512;
+        for (int mt = 0; mt < mtb; ++mt)
+            for (int s = 0; s < ksb; ++s)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int j = 0; j < 8; ++j) {
+                        const int unit = 16 * (2 * s + j / 4) + 4 * (lane / 16) + j % 4;   // output unit of layer k
+                        const int col = 16 * mt + lane % 16;                               // input unit of layer k
+                        const float w = (unit < width && col < kin) ? W[unit * kin + col] : 0.f;
+                        put(L.bwd_off(k), lob, ((mt * ksb + s) * 64 + lane) * 8 + j, w);
+                    }
+    }
+    return NEPX_OK;
+}
+
 extern "C" int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
                                    const int32_t* pair_j, const float* pair_t, const int32_t* np_dev,
                                    int64_t maxp, float* G, void* stream) {
@@ -476,13 +556,26 @@ extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int3
                  net->width, net->n_hidden + 1);
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-#define X(d, w, h, T, NT)                                                                              \
-    if (net->d_in == d && net->width == w && net->n_hidden == h) {                                     \
-        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64, 2 * 256);        \
-        hipLaunchKernelGGL((k_ffnn<T, NT>), dim3(grid), dim3(kBlock), 0, as_stream(stream),            \
-                           net->w_packed, net->act, net->en_min, scale, d, G, np_dev, maxp, E, dEdG);  \
-        NEPX_LAUNCH_CHECK();                                                                           \
-        return NEPX_OK;                                                                                \
+    const bool half = net->precision == NEPX_NET_F16X3;
+    NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
+#define X(d, w, h, T, NT)                                                                                    \
+    if (net->d_in == d && net->width == w && net->n_hidden == h) {                                           \
+        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64, 2 * 256);              \
+        if constexpr (T::HALF_OK) {                                                                          \
+            if (half) {                                                                                      \
+                hipLaunchKernelGGL((k_ffnn<T, NT, true>), dim3(grid), dim3(kBlock), 0, as_stream(stream),    \
+                                   net->w_packed, net->w_half, net->act, net->en_min, scale, d, G, np_dev,   \
+                                   maxp, E, dEdG);                                                           \
+                NEPX_LAUNCH_CHECK();                                                                         \
+                return NEPX_OK;                                                                              \
+            }                                                                                                \
+        }                                                                                                    \
+        NEPX_REQUIRE(!half, NEPX_EUNSUP, "no fp16 network kernel for width %d", w);                          \
+        hipLaunchKernelGGL((k_ffnn<T, NT, false>), dim3(grid), dim3(kBlock), 0, as_stream(stream),           \
+                           net->w_packed, net->w_half, net->act, net->en_min, scale, d, G, np_dev, maxp, E,  \
+                           dEdG);                                                                            \
+        NEPX_LAUNCH_CHECK();                                                                                 \
+        return NEPX_OK;                                                                                      \
     }
     NEPX_NET_CASES(X)
 #undef X
@@ -538,14 +631,22 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
     const int grid = grid_for_cap(maxp, kBlock, 256);
-#define X(ns, a, b, c, T, d, w, h, NT)                                                                \
-    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&         \
-        net->width == w && net->n_hidden == h) {                                                      \
-        hipLaunchKernelGGL((k_fused<T, NT>), dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,       \
-                           net->w_packed, net->act, net->en_min, scale, (const float4*)qf, pair_i,     \
-                           pair_j, (const float4*)pair_t, np_dev, maxp, clip, out);                   \
-        NEPX_LAUNCH_CHECK();                                                                          \
-        return NEPX_OK;                                                                               \
+    const bool half = net->precision == NEPX_NET_F16X3;
+    NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
+#define X(ns, a, b, c, T, d, w, h, NT)                                                                      \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&               \
+        net->width == w && net->n_hidden == h) {                                                            \
+        if (half) {                                                                                         \
+            hipLaunchKernelGGL((k_fused<T, NT, true>), dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,  \
+                               net->w_packed, net->w_half, net->act, net->en_min, scale, (const float4*)qf, \
+                               pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, clip, out);             \
+        } else {                                                                                            \
+            hipLaunchKernelGGL((k_fused<T, NT, false>), dim3(grid), dim3(kBlock), 0, as_stream(stream), *m, \
+                               net->w_packed, net->w_half, net->act, net->en_min, scale, (const float4*)qf, \
+                               pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, clip, out);             \
+        }                                                                                                   \
+        NEPX_LAUNCH_CHECK();                                                                                \
+        return NEPX_OK;                                                                                     \
     }
     NEPX_FUSED_CASES(X)
 #undef X

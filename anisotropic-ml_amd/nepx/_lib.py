@@ -14,6 +14,8 @@ LIB_PATH = os.environ.get("NEPX_LIB", os.path.join(_HERE, "libnepx.so"))
 MAX_SITES = 16
 ACT_TANH = 0
 ACT_RELU = 1
+NET_F32 = 0
+NET_F16X3 = 1
 
 c_i32p = C.POINTER(C.c_int32)
 c_f32p = C.POINTER(C.c_float)
@@ -27,7 +29,8 @@ class PairModel(C.Structure):
 
 class Net(C.Structure):
     _fields_ = [("d_in", C.c_int32), ("width", C.c_int32), ("n_hidden", C.c_int32), ("act", C.c_int32),
-                ("en_min", C.c_float), ("en_max", C.c_float), ("w_packed", C.c_void_p)]
+                ("en_min", C.c_float), ("en_max", C.c_float), ("w_packed", C.c_void_p), ("w_half", C.c_void_p),
+                ("precision", C.c_int32), ("pad", C.c_int32)]
 
 
 _STATE_PTRS = ["x", "y", "z", "vx", "vy", "vz", "q0", "q1", "q2", "q3", "p0", "p1", "p2", "p3", "mass",
@@ -80,6 +83,8 @@ def _load():
         "nepx_net_packed_floats": ([C.c_int32, C.c_int32, C.c_int32], C.c_int64),
         "nepx_net_pack": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(c_f32p), C.POINTER(c_f32p), c_f32p],
                           C.c_int),
+        "nepx_net_packed_halves": ([C.c_int32, C.c_int32, C.c_int32], C.c_int64),
+        "nepx_net_pack_half": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(c_f32p), vp], C.c_int),
         "nepx_descriptor_fwd": ([C.POINTER(PairModel), vp, vp, vp, vp, vp, C.c_int64, vp, vp], C.c_int),
         "nepx_ffnn_fwd_bwd": ([C.POINTER(Net), vp, vp, C.c_int64, vp, vp, vp], C.c_int),
         "nepx_descriptor_adjoint": ([C.POINTER(PairModel), vp, vp, vp, vp, vp, C.c_int64, vp, vp, C.c_float,

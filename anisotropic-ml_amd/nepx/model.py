@@ -38,7 +38,7 @@ class EnergyNet:
     (width)*n_hidden -> 1, activation after every layer but the last.
     """
 
-    def __init__(self, layers, act="tanh", en_min=0.0, en_max=1.0, device="cuda"):
+    def __init__(self, layers, act="tanh", en_min=0.0, en_max=1.0, device="cuda", precision="auto"):
         self.layers = [(np.asarray(W, dtype=np.float32), np.asarray(b, dtype=np.float32).reshape(-1))
                        for W, b in layers]
         self.d_in = self.layers[0][0].shape[1]
@@ -54,18 +54,26 @@ class EnergyNet:
         self.en_max = float(en_max)
         self.device = device
         self.packed = self._pack()
+        self.packed_half = self._pack_half()
+        if precision == "auto":
+            precision = "f16x3" if self.packed_half is not None else "f32"
+        if precision == "f16x3" and self.packed_half is None:
+            raise ValueError("the fp16 split path needs a width whose 16-padding is a multiple of 32")
+        self.precision = precision
         self.w_dev = torch.from_numpy(self.packed).to(device) if device is not None else None
+        self.wh_dev = (torch.from_numpy(self.packed_half.view(np.int16)).to(device)
+                       if device is not None and self.packed_half is not None else None)
 
     @classmethod
-    def random(cls, d_in, widths, seed, act="tanh", en_min=0.0, en_max=1.0, device="cuda"):
+    def random(cls, d_in, widths, seed, act="tanh", en_min=0.0, en_max=1.0, device="cuda", precision="auto"):
         """Seeded N(0, 1/fan_in) weights and zero biases."""
         rng = np.random.default_rng(seed)
         dims = [d_in] + list(widths) + [1]
         layers = [(rng.standard_normal((b, a)) / math.sqrt(a), np.zeros(b)) for a, b in zip(dims[:-1], dims[1:])]
-        return cls(layers, act, en_min, en_max, device)
+        return cls(layers, act, en_min, en_max, device, precision)
 
     @classmethod
-    def from_state_dict(cls, sd, act="relu", en_min=0.0, en_max=1.0, device="cuda"):
+    def from_state_dict(cls, sd, act="relu", en_min=0.0, en_max=1.0, device="cuda", precision="auto"):
         """TheEnergyNet state_dict: layer_first, layer_hidden.k, layer_last."""
         def g(k):
             v = sd[k]
@@ -76,7 +84,7 @@ class EnergyNet:
             layers.append((g(f"layer_hidden.{k}.weight"), g(f"layer_hidden.{k}.bias")))
             k += 1
         layers.append((g("layer_last.weight"), g("layer_last.bias")))
-        return cls(layers, act, en_min, en_max, device)
+        return cls(layers, act, en_min, en_max, device, precision)
 
     @classmethod
     def load(cls, path, **kw):
@@ -98,12 +106,29 @@ class EnergyNet:
                                           out.ctypes.data_as(_lib.c_f32p)), "nepx_net_pack")
         return out
 
+    def _pack_half(self):
+        n = _lib.lib.nepx_net_packed_halves(self.d_in, self.width, self.n_hidden)
+        if n <= 0:
+            return None
+        out = np.zeros(n, dtype=np.float16)
+        ws = [np.ascontiguousarray(W, dtype=np.float32) for W, _ in self.layers]
+        WP = (_lib.c_f32p * len(ws))(*[w.ctypes.data_as(_lib.c_f32p) for w in ws])
+        _lib.check(_lib.lib.nepx_net_pack_half(self.d_in, self.width, self.n_hidden, WP,
+                                               out.ctypes.data_as(C.c_void_p)), "nepx_net_pack_half")
+        return out
+
+    def with_precision(self, precision):
+        """A copy of this network run at another MFMA precision ('f32' or 'f16x3')."""
+        return EnergyNet(self.layers, self.act, self.en_min, self.en_max, self.device, precision)
+
     def cstruct(self):
         n = _lib.Net()
         n.d_in, n.width, n.n_hidden = self.d_in, self.width, self.n_hidden
         n.act = _lib.ACT_TANH if self.act == "tanh" else _lib.ACT_RELU
         n.en_min, n.en_max = self.en_min, self.en_max
         n.w_packed = self.w_dev.data_ptr()
+        n.w_half = self.wh_dev.data_ptr() if self.wh_dev is not None else None
+        n.precision = _lib.NET_F16X3 if self.precision == "f16x3" else _lib.NET_F32
         return n
 
     def layers_f64(self):

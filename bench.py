@@ -19,6 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "anisotropic-ml_amd")]
 
 PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_FP16_MFMA_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -116,6 +117,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="tet32k")
+    ap.add_argument("--precision", default="auto", choices=["auto", "f32", "f16x3"],
+                    help="network MFMA operands: FP32, or fp16 hi/lo split (3 products, FP32 accumulate)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", type=int, default=1, help="capture the step in a hipGraph (1) or launch eagerly (0)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -149,7 +152,7 @@ def main():
     widths = [64, 64] if D == 40 else [128, 128]
     spec = make_system(w["shape"], w["n"], w["phi"], kT=w["kT"], dt=w["dt"], seed=2024)
     dev = torch.device("cuda:0")
-    net = nepx.EnergyNet.random(D, widths, seed=7, device=dev)
+    net = nepx.EnergyNet.random(D, widths, seed=7, device=dev, precision=a.precision)
     ar = nepx.Arena([spec], shape, hyp, net, device=dev)
     runner = StepRunner(ar, use_graph=bool(a.graph))
     runner.init()
@@ -178,6 +181,9 @@ def main():
 
     flops = ffnn_flops_per_pair(D, widths) * P
     achieved = flops / (fused_ms * 1e-3) / 1e12
+    half = net.precision == "f16x3"
+    peak = PEAK_FP16_MFMA_TFLOPS if half else PEAK_FP32_MFMA_TFLOPS
+    mfma_desc = "fp16 hi/lo split MFMA (3 products per FP32 product, FP32 accumulate)" if half else "MFMA f32"
     traffic, traffic_src = profiled_traffic(a.workload)
     res = {
         "metric": "particle-steps/s at 32k tetrahedra",
@@ -190,14 +196,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair math (MFMA f32), f64 integrator",
+        "dtype": f"f32 pair math ({mfma_desc}), f64 integrator",
         "data": "synthetic (seeded lattice + random orientations, random-init FFNN)",
         "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc=2.5 hypers={hyp} D={D} "
                                f"FFNN {D}->{widths[0]}->{widths[1]}->1 tanh, NVT kT={w['kT']} dt={w['dt']}",
                    "particles": spec.n, "pairs_per_step": P, "hipgraph": bool(a.graph),
                    "list_builds": int(flags[4]), "wall_s": wall},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_fused (descriptor + FFNN fwd/bwd + adjoint)", "kernel_ms": fused_ms,
                      "flops_per_launch": flops},
         "thermo": {"Kt_per_dof": float(th["Kt"] / spec.ndof * 2), "U": float(th["U"])},

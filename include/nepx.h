@@ -45,8 +45,14 @@ typedef struct nepx_pair_model {
 typedef struct nepx_net {
     int32_t d_in, width, n_hidden, act;
     float en_min, en_max;            /* E = en_min + (en_max-en_min) * NN(G)    */
-    const float* w_packed;
+    const float* w_packed;           /* nepx_net_pack output (FP32 operands, biases, output layer) */
+    const void* w_half;              /* nepx_net_pack_half output, or NULL        */
+    int32_t precision;               /* NEPX_NET_F32 or NEPX_NET_F16X3           */
+    int32_t pad;
 } nepx_net;
+
+#define NEPX_NET_F32 0               /* v_mfma_f32_16x16x4_f32                    */
+#define NEPX_NET_F16X3 1             /* hi/lo fp16 split, 3 x v_mfma_f32_16x16x32_f16, FP32 accumulate */
 
 /* Per-system parameters (device array of n_sys records, 16 doubles + 16 ints). */
 typedef struct nepx_sysparam {
@@ -128,6 +134,10 @@ int64_t nepx_net_packed_floats(int32_t d_in, int32_t width, int32_t n_hidden);
  * (ForceEvaluator.py:55-61).  ws/bs: n_hidden+2 host pointers each. */
 int nepx_net_pack(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
                   const float* const* bs, float* host_out);
+/* fp16 hi/lo operands for NEPX_NET_F16X3 (width a multiple of 32); count in halves. */
+int64_t nepx_net_packed_halves(int32_t d_in, int32_t width, int32_t n_hidden);
+int nepx_net_pack_half(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
+                       uint16_t* host_out);
 
 /* ---- descriptor, energy network, adjoint ------------------------------- */
 /* G (P x D, f32) of compacted pairs.  Replaces pos_to_pts + pts_to_nep_descriptors

@@ -53,23 +53,25 @@ def test_empty_pair_set(gpu):
     assert G.shape == (0, 40)
 
 
+@pytest.mark.parametrize("precision", ["f32", "f16x3"])
 @pytest.mark.parametrize("d_in,widths,act", [(40, [64, 64], "tanh"), (60, [128, 128], "tanh"),
-                                             (40, [64, 64], "relu")])
-def test_ffnn_matches_oracle(gpu, d_in, widths, act):
+                                             (40, [64, 64], "relu"), (47, [150] * 4, "relu")])
+def test_ffnn_matches_oracle(gpu, d_in, widths, act, precision):
     import nepx
     from nepx import ops
     rng = np.random.default_rng(3)
     P = 1000
     # realistic descriptor magnitudes plus a small-scale block that keeps tanh unsaturated
     G = np.concatenate([rng.uniform(0, 20, (P // 2, d_in)), rng.normal(0, 0.2, (P - P // 2, d_in))]).astype(np.float32)
-    net = nepx.EnergyNet.random(d_in, widths, seed=7, act=act, en_min=-2.0, en_max=3.0, device=gpu)
+    net = nepx.EnergyNet.random(d_in, widths, seed=7, act=act, en_min=-2.0, en_max=3.0, device=gpu,
+                                precision=precision)
     E, dG = ops.ffnn_forward_backward(net, torch.from_numpy(G).to(gpu))
     y, g = O.ffnn_forward_backward(G.astype(np.float64), net.layers_f64(), act)
     E64 = -2.0 + 5.0 * y
     dG64 = 5.0 * g
     eE = normwise_rel(E.cpu().numpy(), E64)
     eG = normwise_rel(dG.cpu().numpy(), dG64)
-    print(f"ffnn {d_in}->{widths} {act}: E {eE:.3e} dG {eG:.3e}")
+    print(f"ffnn {d_in}->{widths} {act} {precision}: E {eE:.3e} dG {eG:.3e}")
     assert eE < 1e-5
     assert eG < 1e-4
 
@@ -97,20 +99,21 @@ def test_adjoint_matches_oracle(gpu, shape_name, hyp):
     assert max(eF, eTi, eTj) < 1e-4
 
 
+@pytest.mark.parametrize("precision", ["f32", "f16x3"])
 @pytest.mark.parametrize("shape_name,hyp,widths", [("tet10", (11, 6, 4), [64, 64]), ("cube8", (11, 6, 4), [64, 64])])
-def test_fused_matches_pipeline(gpu, shape_name, hyp, widths):
+def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision):
     import nepx
     from nepx import ops
     shape, (q, i, j, t), T = _inputs(gpu, shape_name, n_pairs=1000, seed=9)
     model = nepx.make_pair_model(shape, hyp)
     D = nepx.n_descriptor(*hyp)
-    net = nepx.EnergyNet.random(D, widths, seed=2, device=gpu)
+    net = nepx.EnergyNet.random(D, widths, seed=2, device=gpu, precision=precision)
     G = ops.descriptor_forward(model, T["qf"], T["pi"], T["pj"], T["pt"])
     E, dG = ops.ffnn_forward_backward(net, G)
     ref = ops.descriptor_adjoint(model, T["qf"], T["pi"], T["pj"], T["pt"], dG, E=E, clip=100.0).cpu().numpy()
     fused = ops.pair_fused(model, net, T["qf"], T["pi"], T["pj"], T["pt"], clip=100.0).cpu().numpy()
     e = normwise_rel(fused, ref)
-    print(f"fused vs pipeline {shape_name}: {e:.3e}")
+    print(f"fused vs pipeline {shape_name} {precision}: {e:.3e}")
     assert e < 1e-6
     # and against FP64 end to end (pair energy + analytic forces)
     clouds = oracle_clouds(q, i, j, t, shape.sites)

@@ -13,12 +13,12 @@ from oracle import nep_oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def _arena(gpu, shape_name, n, phi, seed=0, skin=0.5, widths=(64, 64), hyp=(11, 6, 4)):
+def _arena(gpu, shape_name, n, phi, seed=0, skin=0.5, widths=(64, 64), hyp=(11, 6, 4), precision="auto"):
     import nepx
     from nepx.configs import make_system
     spec = make_system(shape_name, n, phi, seed=seed, skin=skin)
     D = nepx.n_descriptor(*hyp)
-    net = nepx.EnergyNet.random(D, list(widths), seed=seed + 100, device=gpu)
+    net = nepx.EnergyNet.random(D, list(widths), seed=seed + 100, device=gpu, precision=precision)
     ar = nepx.Arena([spec], nepx.get_shape(shape_name), hyp, net, device=gpu)
     return ar, spec, net
 
@@ -44,8 +44,9 @@ def test_pair_set_bit_exact(gpu, shape_name, n, phi):
     assert np.array_equal(got, want)
 
 
-def test_system_forces_match_oracle(gpu):
-    ar, spec, net = _arena(gpu, "tet10", 400, 0.1, seed=4)
+@pytest.mark.parametrize("precision", ["f32", "f16x3"])
+def test_system_forces_match_oracle(gpu, precision):
+    ar, spec, net = _arena(gpu, "tet10", 400, 0.1, seed=4, precision=precision)
     ar.init()
     ar.filter_pairs()
     ar.pair_fused()
@@ -57,7 +58,7 @@ def test_system_forces_match_oracle(gpu):
     eF = np.max(np.abs(got["F"] - F)) / np.max(np.abs(F))
     eT = np.max(np.abs(got["T"] - T)) / np.max(np.abs(T))
     eU = abs(got["upot"].sum() - r["E"].sum()) / np.sum(np.abs(r["E"]))
-    print(f"system forces: {len(pairs)} pairs, F {eF:.2e} T {eT:.2e} U {eU:.2e}")
+    print(f"system forces ({precision}): {len(pairs)} pairs, F {eF:.2e} T {eT:.2e} U {eU:.2e}")
     assert eF < 1e-4 and eT < 1e-4 and eU < 1e-5
     # momentum: Newton's third law holds pair by pair
     assert np.abs(got["F"].sum(0)).max() < 1e-9 * max(1.0, np.abs(got["F"]).sum())

@@ -92,6 +92,90 @@ def test_net_packing_reproduces_dense_network(d_in, width, n_hidden, act):
     assert np.allclose(dG, g64, rtol=1e-4, atol=1e-5)
 
 
+def _split16(x):
+    hi = x.astype(np.float16)
+    return hi.astype(np.float64), (x - hi.astype(np.float32)).astype(np.float16).astype(np.float64)
+
+
+def _emulate_f16x3_net(half, packed, d_in, width, n_hidden, G, act):
+    """Run the fp16 hi/lo operands through the 16x16x32 f16 MFMA lane layout on the CPU.
+
+    A operand lane l, element j: A[l%16][8*(l/16) + j]; B operand likewise with
+    the column l%16.  K-step s of a layer maps (lane group g, j) to unit
+    16*(2s + j/4) + 4g + j%4.  The activations are split into fp16 hi/lo parts
+    exactly as on the device (deltas after scaling by 2^10) and the products
+    hi*hi + hi*lo + lo*hi are summed, so both the operand layout and the
+    precision of the split are checked.
+    """
+    DP, HP, NL = (d_in + 15) // 16 * 16, (width + 15) // 16 * 16, n_hidden + 1
+    DPH = (DP + 31) // 32 * 32
+    fsz = [2 * (HP // 16) * ((DPH if k == 0 else HP) // 32) * 512 for k in range(NL)]
+    bsz = [2 * ((DP if k == 0 else HP) // 16) * (HP // 32) * 512 for k in range(NL)]
+    foff = [sum(fsz[:k]) for k in range(NL)]
+    boff = [sum(fsz) + sum(bsz[:k]) for k in range(NL)]
+    # fp32 layout: biases and the output layer
+    f32 = [(HP // 16) * ((DP if k == 0 else HP) // 4) * 64 for k in range(NL)]
+    b32 = [((DP if k == 0 else HP) // 16) * (HP // 4) * 64 for k in range(NL)]
+    bias_off = [sum(f32) + sum(b32) + k * HP for k in range(NL)]
+    wout = bias_off[-1] + HP
+    H = half.astype(np.float64)
+
+    def matmul(off, M, K, X, sc):
+        ks = K // 32
+        lo = (M // 16) * ks * 512
+        xh, xl = _split16((X * sc).astype(np.float32))
+        out = np.zeros((M, X.shape[1]))
+        for mt in range(M // 16):
+            for s in range(ks):
+                for lane in range(64):
+                    for j in range(8):
+                        e = off + ((mt * ks + s) * 64 + lane) * 8 + j
+                        row = 16 * mt + lane % 16
+                        u = 16 * (2 * s + j // 4) + 4 * (lane // 16) + j % 4
+                        out[row] += H[e] * xh[u] + H[e] * xl[u] + H[e + lo] * xh[u]
+        return out / sc
+
+    Gp = np.zeros((G.shape[0], DPH))
+    Gp[:, :d_in] = G
+    h = Gp.T
+    ders = []
+    for k in range(NL):
+        K = DPH if k == 0 else HP
+        z = matmul(foff[k], HP, K, h if k else h, 1.0) + packed[bias_off[k]:bias_off[k] + HP][:, None]
+        h = np.tanh(z) if act == "tanh" else np.maximum(z, 0)
+        ders.append(1 - h * h if act == "tanh" else (h > 0).astype(float))
+    y = packed[wout:wout + HP] @ h + packed[wout + HP]
+    delta = packed[wout:wout + HP][:, None] * ders[-1]
+    for k in range(NL - 1, 0, -1):
+        delta = matmul(boff[k], HP, HP, delta, 1024.0) * ders[k - 1]
+    dG = matmul(boff[0], DP, HP, delta, 1024.0)
+    return y, dG[:d_in].T
+
+
+@pytest.mark.parametrize("d_in,width,n_hidden,act", [(40, 64, 1, "tanh"), (60, 128, 1, "tanh")])
+def test_half_net_packing_and_precision(d_in, width, n_hidden, act):
+    import nepx
+    net = nepx.EnergyNet.random(d_in, [width] * (n_hidden + 1), seed=5, act=act, device=None)
+    assert net.precision == "f16x3"
+    rng = np.random.default_rng(1)
+    G = rng.uniform(0, 2, (4, d_in)) * rng.choice([1.0, 1e-3], size=d_in)   # include small descriptor entries
+    y, dG = _emulate_f16x3_net(net.packed_half, net.packed.astype(np.float64), d_in, width, n_hidden, G, act)
+    y64, g64 = O.ffnn_forward_backward(G, net.layers_f64(), act)
+    assert np.abs(y - y64).max() < 1e-5 * np.abs(y64).max()
+    assert np.abs(dG - g64).max() < 1e-4 * np.abs(g64).max()
+
+
+def test_half_net_needs_width_multiple_of_32():
+    import nepx
+    from nepx import _lib
+    assert _lib.lib.nepx_net_packed_halves(47, 150, 3) > 0          # pads to 160
+    assert _lib.lib.nepx_net_packed_halves(40, 48, 1) == -1         # pads to 48
+    net = nepx.EnergyNet.random(40, [48, 48], seed=1, device=None)
+    assert net.precision == "f32"
+    with pytest.raises(ValueError):
+        nepx.EnergyNet.random(40, [48, 48], seed=1, device=None, precision="f16x3")
+
+
 def test_net_pack_rejects_bad_input():
     from nepx import _lib
     assert _lib.lib.nepx_net_pack(0, 64, 1, None, None, None) == -1
