@@ -29,26 +29,19 @@ struct HarmTables {
     __host__ __device__ static constexpr int qi(int l, int m) { return l * (l + 1) / 2 + m; }
 };
 
-// K_lm = sqrt(2 (l-m)!/(l+m)!) for m > 0 (1 for m = 0), evaluated at compile
-// time; index l*(LMAX+1)+m.  Used through a constexpr local so every lookup
-// folds to an immediate once the (l, m) loops are unrolled.
-template <int LMAX>
-struct KTab {
-    float v[(LMAX + 1) * (LMAX + 1)];
-    constexpr KTab() : v() {
-        for (int l = 0; l <= LMAX; ++l)
-            for (int m = 0; m <= l; ++m)
-                v[l * (LMAX + 1) + m] = m == 0 ? 1.0f : (float)csqrt(2.0 * cfact(l - m) / cfact(l + m));
-    }
-    __host__ __device__ constexpr float operator()(int l, int m) const { return v[l * (LMAX + 1) + m]; }
-};
-
-// Q_l^m(z) at |u| = 1, cos/sin of m phi times sin^m theta, and Yhat.
+// Legendre parts Q_l^m(z) at |u| = 1 (Condon-Shortley phase) and
+// C_m + i S_m = (x + iy)^m.  The unnormalised solid harmonics are
+//   Z_l0 = Q_l^0,  Z_lm^c = Q_l^m C_m,  Z_lm^s = Q_l^m S_m   (m >= 1)
+// and Yhat_lm = K_lm Z_lm.  The descriptor kernels accumulate A' = sum f Z
+// and weight by K_lm^2 where the squares are formed, so no per-site
+// multiplication by K is needed.
 template <int LMAX>
 struct Harm {
     using T = HarmTables<LMAX>;
     float Q[T::NQ];
     float C[LMAX + 1], S[LMAX + 1];
+
+    __device__ __forceinline__ float q(int l, int m) const { return Q[T::qi(l, m)]; }
 
     __device__ __forceinline__ void eval(float x, float y, float z) {
         Q[0] = 1.f;
@@ -73,22 +66,32 @@ struct Harm {
             S[m] = x * S[m - 1] + y * C[m - 1];
         }
     }
-    // Yhat for l >= 1, index l*l-1 + {0: m=0, 2m-1: cos m, 2m: sin m}
-    __device__ __forceinline__ void yhat(float* Y) const {
-        constexpr KTab<LMAX> kfac{};
-#pragma unroll
-        for (int l = 1; l <= LMAX; ++l) {
-            const int o = l * l - 1;
-            Y[o] = Q[T::qi(l, 0)];
-#pragma unroll
-            for (int m = 1; m <= l; ++m) {
-                const float kq = kfac(l, m) * Q[T::qi(l, m)];
-                Y[o + 2 * m - 1] = kq * C[m];
-                Y[o + 2 * m] = kq * S[m];
-            }
-        }
-    }
 };
+
+// K_lm^2 = 2 (l-m)!/(l+m)! (m > 0), 1 (m = 0), at compile time.
+template <int LMAX>
+struct K2Tab {
+    float v[(LMAX + 1) * (LMAX + 1)];
+    constexpr K2Tab() : v() {
+        for (int l = 0; l <= LMAX; ++l)
+            for (int m = 0; m <= l; ++m)
+                v[l * (LMAX + 1) + m] = m == 0 ? 1.0f : (float)(2.0 * cfact(l - m) / cfact(l + m));
+    }
+    __host__ __device__ constexpr float operator()(int l, int m) const { return v[l * (LMAX + 1) + m]; }
+};
+
+// Floats of angular accumulators one lane holds in a pass over the sites.
+constexpr int kAccBudget = 176;
+constexpr int split_l(int nang, int lmax) {
+    int L = lmax;
+    while (L > 1 && (nang + 1) * L * (L + 2) > kAccBudget) --L;
+    return L;
+}
+
+// Slot of (l, m, component) in the NLM accumulators: l*l-1 + {0: m = 0, 2m-1: cos, 2m: sin}.
+__host__ __device__ constexpr int lm_slot(int l, int m, int sin_part) {
+    return l * l - 1 + (m == 0 ? 0 : 2 * m - 1 + sin_part);
+}
 
 template <int NS_, int NRAD_, int NANG_, int LMAX_>
 struct DescCfg {
@@ -97,7 +100,25 @@ struct DescCfg {
     static constexpr int NLM = LMAX * (LMAX + 2);
     static constexpr int D = (NRAD + 1) + (NANG + 1) * LMAX;
     static_assert(NANG <= NRAD, "nang must not exceed nrad");
+    // The angular accumulators of l = 1..L_SPLIT are held per lane in one
+    // pass; any higher l go to a second pass over the sites.
+    static constexpr int L_SPLIT = split_l(NANG, LMAX);
+    static constexpr bool ONE_PASS = L_SPLIT == LMAX;
+    static_assert(ONE_PASS || (NANG + 1) * (LMAX * (LMAX + 2) - L_SPLIT * (L_SPLIT + 2)) <= 2 * kAccBudget,
+                  "second l-range too wide");
 };
+
+// A contiguous range of l: accumulator slots lm_slot(l, m, part) - OFF.
+template <int L0_, int L1_>
+struct LRange {
+    static constexpr int L0 = L0_, L1 = L1_;
+    static constexpr int OFF = L0 * L0 - 1;
+    static constexpr int N = (L1 + 1) * (L1 + 1) - L0 * L0;
+};
+template <class Cfg>
+using Pass0 = LRange<1, Cfg::L_SPLIT>;
+template <class Cfg>
+using Pass1 = LRange<Cfg::L_SPLIT + 1, Cfg::LMAX>;
 
 // One pair in the frame centred on particle i: orientations, t = X_j - X_i,
 // and the centroid of the 2*Ns sites,
@@ -154,17 +175,20 @@ __device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3
     return s;
 }
 
-// Forward accumulators: g_rad[n] and A[n][lm] = sum_a f_n(a) Yhat_lm(u_a).
-template <class Cfg>
+// Forward accumulators over the l of range LR: A[n][lm - LR::OFF] = sum_a f_n(a) Z_lm(u_a),
+// and (RAD) g_rad[n] = sum_a f_n(a).
+template <class Cfg, class LR, bool RAD>
 __device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const PairIn& p,
                                                 float (&grad)[Cfg::NRAD + 1],
-                                                float (&A)[Cfg::NANG + 1][Cfg::NLM]) {
+                                                float (&A)[Cfg::NANG + 1][LR::N]) {
+    if constexpr (RAD) {
 #pragma unroll
-    for (int n = 0; n <= Cfg::NRAD; ++n) grad[n] = 0.f;
+        for (int n = 0; n <= Cfg::NRAD; ++n) grad[n] = 0.f;
+    }
 #pragma unroll
     for (int n = 0; n <= Cfg::NANG; ++n)
 #pragma unroll
-        for (int k = 0; k < Cfg::NLM; ++k) A[n][k] = 0.f;
+        for (int k = 0; k < LR::N; ++k) A[n][k] = 0.f;
     const float inv_rn = 1.f / m.r_nep;
 #pragma unroll 1
     for (int body = 0; body < 2; ++body) {
@@ -176,16 +200,11 @@ __device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const 
         for (int k = 0; k < Cfg::NS; ++k) {
             const SiteGeo s = site_geo(m, R, k, ox, oy, oz, inv_rn);
             const float x = 2.f * (s.rho - 1.f) * (s.rho - 1.f) - 1.f;
-            const float fc = 0.5f * (1.f + cospif(s.rho));
-            float Y[Cfg::NLM];
-            {
-                Harm<Cfg::LMAX> h;
-                h.eval(s.ux, s.uy, s.uz);
-                h.yhat(Y);
-            }
+            const float hf = 0.25f * (1.f + cospif(s.rho));  // fc / 2
+            float fa[Cfg::NANG + 1];
             float t0 = 1.f, t1 = x;
 #pragma unroll
-            for (int n = 0; n <= Cfg::NRAD; ++n) {
+            for (int n = 0; n <= (RAD ? Cfg::NRAD : Cfg::NANG); ++n) {
                 float tn;
                 if (n == 0) tn = 1.f;
                 else if (n == 1) tn = x;
@@ -194,30 +213,45 @@ __device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const 
                     t0 = t1;
                     t1 = tn;
                 }
-                const float f = 0.5f * (tn + 1.f) * fc;
-                grad[n] += f;
-                if (n <= Cfg::NANG) {
+                const float f = tn * hf + hf;
+                if constexpr (RAD) grad[n] += f;
+                if (n <= Cfg::NANG) fa[n] = f;
+            }
+            Harm<LR::L1> h;
+            h.eval(s.ux, s.uy, s.uz);
 #pragma unroll
-                    for (int kk = 0; kk < Cfg::NLM; ++kk) A[n][kk] += f * Y[kk];
+            for (int l = LR::L0; l <= LR::L1; ++l) {
+#pragma unroll
+                for (int mm = 0; mm <= l; ++mm) {
+                    const float ql = h.q(l, mm);
+#pragma unroll
+                    for (int part = 0; part < (mm ? 2 : 1); ++part) {
+                        const float z = mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]);
+                        const int o = lm_slot(l, mm, part) - LR::OFF;
+#pragma unroll
+                        for (int n = 0; n <= Cfg::NANG; ++n) A[n][o] += fa[n] * z;
+                    }
                 }
             }
         }
     }
 }
 
-template <class Cfg>
-__device__ __forceinline__ void desc_finish(const float (&grad)[Cfg::NRAD + 1],
-                                            const float (&A)[Cfg::NANG + 1][Cfg::NLM],
-                                            float (&G)[Cfg::D]) {
-#pragma unroll
-    for (int n = 0; n <= Cfg::NRAD; ++n) G[n] = grad[n];
+// G entries of range LR: g_ang[n][l] = sum_m K_lm^2 A[n][lm]^2, at G[NRAD+1 + n*LMAX + l-1].
+template <class Cfg, class LR, class Row>
+__device__ __forceinline__ void desc_finish(const float (&A)[Cfg::NANG + 1][LR::N], Row&& G) {
+    constexpr K2Tab<Cfg::LMAX> k2{};
 #pragma unroll
     for (int n = 0; n <= Cfg::NANG; ++n) {
 #pragma unroll
-        for (int l = 1; l <= Cfg::LMAX; ++l) {
-            float s = 0.f;
+        for (int l = LR::L0; l <= LR::L1; ++l) {
+            const float a0 = A[n][lm_slot(l, 0, 0) - LR::OFF];
+            float s = a0 * a0;
 #pragma unroll
-            for (int k = l * l - 1; k < (l + 1) * (l + 1) - 1; ++k) s += A[n][k] * A[n][k];
+            for (int mm = 1; mm <= l; ++mm) {
+                const float ac = A[n][lm_slot(l, mm, 0) - LR::OFF], as = A[n][lm_slot(l, mm, 1) - LR::OFF];
+                s += k2(l, mm) * (ac * ac + as * as);
+            }
             G[Cfg::NRAD + 1 + n * Cfg::LMAX + (l - 1)] = s;
         }
     }
@@ -230,34 +264,46 @@ struct PairOut {
     float tjx, tjy, tjz;  // torque on j about X_j
 };
 
+// Per-body sums of dE/dr_a, arm x dE/dr_a and arms (body 0 = i, 1 = j), summed over passes.
+struct AdjAcc {
+    float G0x = 0.f, G0y = 0.f, G0z = 0.f, T0x = 0.f, T0y = 0.f, T0z = 0.f, A0x = 0.f, A0y = 0.f, A0z = 0.f;
+    float G1x = 0.f, G1y = 0.f, G1z = 0.f, T1x = 0.f, T1y = 0.f, T1z = 0.f, A1x = 0.f, A1y = 0.f, A1z = 0.f;
+};
+
 // Adjoint.  dG = dE/dG = [dE/dg_rad | dE/dg_ang] (already in energy units).  A is
-// the forward accumulator, overwritten by B = dE/dA = 2 dE/dg_ang[n,l] A.
-// Per site a (r_a relative to the centroid, u_a = r_a/|r_a|):
-//   dE/df_n(a) = dgr[n] + sum_lm B_nlm Yhat_lm(u_a)            (n <= nang)
-//   c_lm(a)    = sum_n B_nlm f_n(a)
-//   dE/dr_a    = (sum_n dE/df_n f_n'(rho)) u_a / r_nep
-//              + (1/|r_a|) [ sum_lm c_lm grad S_lm(u_a) - u_a sum_l l sum_m c_lm Yhat_lm(u_a) ]
-// (S_lm: the homogeneous polynomial with S_lm(u) = Yhat_lm(u) on |u| = 1), and
-// dE/dp_a = dE/dr_a - mean_b dE/dr_b through the centroid.
-template <class Cfg>
-__device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const PairIn& p,
-                                                const float (&dG)[Cfg::D],
-                                                float (&A)[Cfg::NANG + 1][Cfg::NLM]) {
+// the forward accumulator, overwritten by B = dE/dA = 2 K_lm^2 dE/dg_ang[n,l] A.
+// Per site a (r_a relative to the centroid, u_a = r_a/|r_a|), with
+//   c_lm(a) = sum_n B_nlm f_n(a),   d_lm(a) = sum_n B_nlm f_n'(rho_a):
+//   dE/drho_a = sum_n dE/dg_rad[n] f_n'(rho_a) + sum_lm d_lm Z_lm(u_a)
+//   dE/dr_a   = dE/drho_a u_a / r_nep
+//             + (1/|r_a|) [ grad sum_lm c_lm Z_lm (u_a) - u_a sum_lm l c_lm Z_lm(u_a) ]
+// where Z_lm is extended to the homogeneous polynomial of degree l, whose
+// gradient follows from
+//   d/dx Q_l^m = x Q_{l-1}^{m+1},  d/dy Q_l^m = y Q_{l-1}^{m+1},  d/dz Q_l^m = (l+m) Q_{l-1}^m,
+//   d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}).
+// The sums over l are taken first for each m, so each (l, m) costs a few FMAs.
+// Finally dE/dp_a = dE/dr_a - mean_b dE/dr_b through the centroid (adj_finish).
+// One pass handles the l of range LR; the radial terms and the arms go with RAD.
+template <class Cfg, class LR, bool RAD>
+__device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const PairIn& p,
+                                             const float (&dG)[Cfg::D],
+                                             float (&A)[Cfg::NANG + 1][LR::N], AdjAcc& acc) {
+    constexpr K2Tab<Cfg::LMAX> k2{};
 #pragma unroll
     for (int n = 0; n <= Cfg::NANG; ++n) {
 #pragma unroll
-        for (int l = 1; l <= Cfg::LMAX; ++l) {
+        for (int l = LR::L0; l <= LR::L1; ++l) {
             const float w = 2.f * dG[Cfg::NRAD + 1 + n * Cfg::LMAX + (l - 1)];
+            A[n][lm_slot(l, 0, 0) - LR::OFF] *= w;
 #pragma unroll
-            for (int k = l * l - 1; k < (l + 1) * (l + 1) - 1; ++k) A[n][k] *= w;
+            for (int mm = 1; mm <= l; ++mm) {
+                const float wk = w * k2(l, mm);
+                A[n][lm_slot(l, mm, 0) - LR::OFF] *= wk;
+                A[n][lm_slot(l, mm, 1) - LR::OFF] *= wk;
+            }
         }
     }
-    using HT = HarmTables<Cfg::LMAX>;
-    constexpr KTab<Cfg::LMAX> kfac{};
     const float inv_rn = 1.f / m.r_nep;
-    // per-body sums of dE/dr_a, arm x dE/dr_a and arms (body 0 = i, 1 = j)
-    float G0x = 0.f, G0y = 0.f, G0z = 0.f, T0x = 0.f, T0y = 0.f, T0z = 0.f, A0x = 0.f, A0y = 0.f, A0z = 0.f;
-    float G1x = 0.f, G1y = 0.f, G1z = 0.f, T1x = 0.f, T1y = 0.f, T1z = 0.f, A1x = 0.f, A1y = 0.f, A1z = 0.f;
 #pragma unroll 1
     for (int body = 0; body < 2; ++body) {
         const float4 q = body ? p.qj : p.qi;
@@ -271,25 +317,18 @@ __device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const 
             const SiteGeo s = site_geo(m, R, k, ox, oy, oz, inv_rn);
             const float xm = s.rho - 1.f;
             const float x = 2.f * xm * xm - 1.f;
-            const float dx = 4.f * xm;
             float sp, cp;
             sincospif(s.rho, &sp, &cp);
-            const float fc = 0.5f * (1.f + cp);
-            const float dfc = -0.5f * (float)M_PI * sp;
-            float c[Cfg::NLM];
-            float dEdrho = 0.f, proj = 0.f;
+            const float hf = 0.25f * (1.f + cp);                      // fc / 2
+            const float hdx = 2.f * xm * (0.5f * (1.f + cp));          // (dx/drho) fc / 2
+            const float hdf = -0.25f * (float)M_PI * sp;               // fc' / 2
+            // f_n = (T_n + 1) fc/2,  f_n' = T_n' x' fc/2 + (T_n + 1) fc'/2
+            float fa[Cfg::NANG + 1], da[Cfg::NANG + 1];
+            float dEdrho = 0.f;
             {
-                float Y[Cfg::NLM];
-                {
-                    Harm<Cfg::LMAX> h;
-                    h.eval(s.ux, s.uy, s.uz);
-                    h.yhat(Y);
-                }
-#pragma unroll
-                for (int kk = 0; kk < Cfg::NLM; ++kk) c[kk] = 0.f;
                 float t0 = 1.f, t1 = x, d0 = 0.f, d1 = 1.f;
 #pragma unroll
-                for (int n = 0; n <= Cfg::NRAD; ++n) {
+                for (int n = 0; n <= (RAD ? Cfg::NRAD : Cfg::NANG); ++n) {
                     float tn, dn;
                     if (n == 0) {
                         tn = 1.f;
@@ -305,50 +344,56 @@ __device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const 
                         d0 = d1;
                         d1 = dn;
                     }
-                    const float f = 0.5f * (tn + 1.f) * fc;
-                    const float df = 0.5f * dn * dx * fc + 0.5f * (tn + 1.f) * dfc;
-                    float sn = dG[n];
+                    const float df = dn * hdx + (tn * hdf + hdf);
+                    if constexpr (RAD) dEdrho += dG[n] * df;
                     if (n <= Cfg::NANG) {
-#pragma unroll
-                        for (int kk = 0; kk < Cfg::NLM; ++kk) {
-                            sn += A[n][kk] * Y[kk];
-                            c[kk] += A[n][kk] * f;
-                        }
+                        fa[n] = tn * hf + hf;
+                        da[n] = df;
                     }
-                    dEdrho += sn * df;
                 }
-#pragma unroll
-                for (int l = 1; l <= Cfg::LMAX; ++l)
-#pragma unroll
-                    for (int kk = l * l - 1; kk < (l + 1) * (l + 1) - 1; ++kk) proj += (float)l * c[kk] * Y[kk];
             }
-            // gradient of sum_lm c_lm S_lm at u (harmonic recurrences recomputed)
-            Harm<Cfg::LMAX> h;
+            Harm<LR::L1> h;
             h.eval(s.ux, s.uy, s.uz);
-            float gx = 0.f, gy = 0.f, gz = 0.f;
+            float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f, proj = 0.f;
 #pragma unroll
-            for (int l = 1; l <= Cfg::LMAX; ++l) {
-                const int o = l * l - 1;
-                {  // m = 0
-                    const float cc = c[o];
-                    const float qa = (l - 1 >= 1) ? h.Q[HT::qi(l - 1, 1)] : 0.f;
-                    gx += cc * s.ux * qa;
-                    gy += cc * s.uy * qa;
-                    gz += cc * (float)l * h.Q[HT::qi(l - 1, 0)];
+            for (int mm = 0; mm <= LR::L1; ++mm) {
+                float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
+#pragma unroll
+                for (int l = (mm > LR::L0 ? mm : LR::L0); l <= LR::L1; ++l) {
+                    const float ql = h.q(l, mm);
+                    const float qa = (mm + 1 <= l - 1) ? h.q(l - 1, mm + 1) : 0.f;
+                    const float qc = (mm <= l - 1) ? (float)(l + mm) * h.q(l - 1, mm) : 0.f;
+                    float pl = 0.f;
+#pragma unroll
+                    for (int part = 0; part < (mm ? 2 : 1); ++part) {
+                        const int o = lm_slot(l, mm, part) - LR::OFF;
+                        const float z = mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]);
+                        float c = 0.f, d = 0.f;
+#pragma unroll
+                        for (int n = 0; n <= Cfg::NANG; ++n) {
+                            c += A[n][o] * fa[n];
+                            d += A[n][o] * da[n];
+                        }
+                        dEdrho += d * z;
+                        pl += c * z;
+                        if (mm + 1 <= l - 1) U[part] += qa * c;
+                        if (mm <= l - 1) V[part] += qc * c;
+                        if (mm >= 1) W[part] += ql * c;
+                    }
+                    proj += (float)l * pl;
                 }
-#pragma unroll
-                for (int mm = 1; mm <= l; ++mm) {
-                    const float K = kfac(l, mm);
-                    const float cc = c[o + 2 * mm - 1] * K, cs = c[o + 2 * mm] * K;
-                    const float qa = (mm + 1 <= l - 1) ? h.Q[HT::qi(l - 1, mm + 1)] : 0.f;
-                    const float qb = h.Q[HT::qi(l, mm)];
-                    const float qc = (mm <= l - 1) ? h.Q[HT::qi(l - 1, mm)] : 0.f;
-                    const float e = cc * h.C[mm] + cs * h.S[mm];
-                    gx += s.ux * qa * e + (float)mm * qb * (cc * h.C[mm - 1] + cs * h.S[mm - 1]);
-                    gy += s.uy * qa * e + (float)mm * qb * (cs * h.C[mm - 1] - cc * h.S[mm - 1]);
-                    gz += (float)(l + mm) * qc * e;
+                if (mm == 0) {
+                    E1 += U[0];
+                    Gz += V[0];
+                } else {
+                    E1 += h.C[mm] * U[0] + h.S[mm] * U[1];
+                    Gz += h.C[mm] * V[0] + h.S[mm] * V[1];
+                    const float fm = (float)mm;
+                    X1 += fm * (h.C[mm - 1] * W[0] + h.S[mm - 1] * W[1]);
+                    Y1 += fm * (h.C[mm - 1] * W[1] - h.S[mm - 1] * W[0]);
                 }
             }
+            const float gx = s.ux * E1 + X1, gy = s.uy * E1 + Y1, gz = Gz;
             const float rad = dEdrho * inv_rn;
             const float ga_x = rad * s.ux + s.ir * (gx - proj * s.ux);
             const float ga_y = rad * s.uy + s.ir * (gy - proj * s.uy);
@@ -359,29 +404,39 @@ __device__ __forceinline__ PairOut desc_adjoint(const nepx_pair_model& m, const 
             tx_s += s.ay * ga_z - s.az * ga_y;
             ty_s += s.az * ga_x - s.ax * ga_z;
             tz_s += s.ax * ga_y - s.ay * ga_x;
-            ax_s += s.ax;
-            ay_s += s.ay;
-            az_s += s.az;
+            if constexpr (RAD) {
+                ax_s += s.ax;
+                ay_s += s.ay;
+                az_s += s.az;
+            }
         }
         if (body == 0) {
-            G0x = gx_s; G0y = gy_s; G0z = gz_s; T0x = tx_s; T0y = ty_s; T0z = tz_s; A0x = ax_s; A0y = ay_s; A0z = az_s;
+            acc.G0x += gx_s; acc.G0y += gy_s; acc.G0z += gz_s;
+            acc.T0x += tx_s; acc.T0y += ty_s; acc.T0z += tz_s;
+            acc.A0x += ax_s; acc.A0y += ay_s; acc.A0z += az_s;
         } else {
-            G1x = gx_s; G1y = gy_s; G1z = gz_s; T1x = tx_s; T1y = ty_s; T1z = tz_s; A1x = ax_s; A1y = ay_s; A1z = az_s;
+            acc.G1x += gx_s; acc.G1y += gy_s; acc.G1z += gz_s;
+            acc.T1x += tx_s; acc.T1y += ty_s; acc.T1z += tz_s;
+            acc.A1x += ax_s; acc.A1y += ay_s; acc.A1z += az_s;
         }
     }
-    // dE/dp_a = g_a - gbar ; F_i = -sum_{a in i} dE/dp_a ; tau = -sum arm x dE/dp
+}
+
+// dE/dp_a = g_a - gbar ; F_i = -sum_{a in i} dE/dp_a ; tau = -sum arm x dE/dp
+template <class Cfg>
+__device__ __forceinline__ PairOut adj_finish(const AdjAcc& a) {
     const float inv_nd = 1.f / (float)Cfg::ND;
-    const float bx = (G0x + G1x) * inv_nd, by = (G0y + G1y) * inv_nd, bz = (G0z + G1z) * inv_nd;
+    const float bx = (a.G0x + a.G1x) * inv_nd, by = (a.G0y + a.G1y) * inv_nd, bz = (a.G0z + a.G1z) * inv_nd;
     PairOut o;
-    o.fx = -(G0x - (float)Cfg::NS * bx);
-    o.fy = -(G0y - (float)Cfg::NS * by);
-    o.fz = -(G0z - (float)Cfg::NS * bz);
-    o.tix = -(T0x - (A0y * bz - A0z * by));
-    o.tiy = -(T0y - (A0z * bx - A0x * bz));
-    o.tiz = -(T0z - (A0x * by - A0y * bx));
-    o.tjx = -(T1x - (A1y * bz - A1z * by));
-    o.tjy = -(T1y - (A1z * bx - A1x * bz));
-    o.tjz = -(T1z - (A1x * by - A1y * bx));
+    o.fx = -(a.G0x - (float)Cfg::NS * bx);
+    o.fy = -(a.G0y - (float)Cfg::NS * by);
+    o.fz = -(a.G0z - (float)Cfg::NS * bz);
+    o.tix = -(a.T0x - (a.A0y * bz - a.A0z * by));
+    o.tiy = -(a.T0y - (a.A0z * bx - a.A0x * bz));
+    o.tiz = -(a.T0z - (a.A0x * by - a.A0y * bx));
+    o.tjx = -(a.T1x - (a.A1y * bz - a.A1z * by));
+    o.tjy = -(a.T1y - (a.A1z * bx - a.A1x * bz));
+    o.tjz = -(a.T1z - (a.A1x * by - a.A1y * bx));
     return o;
 }
 

@@ -77,8 +77,22 @@ struct NetCfg {
     static constexpr NetLayoutH layout_h() { return NetLayoutH{DPH, DP, HP, NL}; }
 };
 
+// tanh from its definition, branch-free: for |z| >= 1/4, (1 - t)/(1 + t) with
+// t = e^{-2|z|} (hardware exp2 and reciprocal; relative error a few 1e-7),
+// below 1/4 the Taylor series z - z^3/3 + 2z^5/15 - 17z^7/315 + 62z^9/2835
+// (truncation error below 1e-8 relative).
+__device__ __forceinline__ float tanh_act(float z) {
+    const float a = fabsf(z);
+    const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * a);  // -2 log2(e) |z|
+    const float r = (1.f - t) * __builtin_amdgcn_rcpf(1.f + t);
+    const float z2 = z * z;
+    const float p =
+        z * (1.f + z2 * (-1.f / 3.f + z2 * (2.f / 15.f + z2 * (-17.f / 315.f + z2 * (62.f / 2835.f)))));
+    return a < 0.25f ? p : copysignf(r, z);
+}
+
 __device__ __forceinline__ float act_fwd(float z, int act) {
-    return act == NEPX_ACT_TANH ? tanhf(z) : fmaxf(z, 0.f);
+    return act == NEPX_ACT_TANH ? tanh_act(z) : fmaxf(z, 0.f);
 }
 __device__ __forceinline__ float act_der(float h, int act) {  // in terms of the output
     return act == NEPX_ACT_TANH ? 1.f - h * h : (h > 0.f ? 1.f : 0.f);

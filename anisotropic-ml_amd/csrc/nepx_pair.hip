@@ -68,6 +68,34 @@ __device__ __forceinline__ void store_out(float* __restrict__ out, int idx, cons
     o4[2] = make_float4(clipf(o.tjx, clip), clipf(o.tjy, clip), clipf(o.tjz, clip), 0.f);
 }
 
+// Descriptor row of one pair, in one or two passes over the sites.
+template <class Cfg, class Row>
+__device__ __forceinline__ void desc_forward(const nepx_pair_model& m, const PairIn& g, Row&& G) {
+    float grad[Cfg::NRAD + 1];
+    {
+        float A[Cfg::NANG + 1][Pass0<Cfg>::N];
+        desc_accumulate<Cfg, Pass0<Cfg>, true>(m, g, grad, A);
+        desc_finish<Cfg, Pass0<Cfg>>(A, G);
+    }
+#pragma unroll
+    for (int n = 0; n <= Cfg::NRAD; ++n) G[n] = grad[n];
+    if constexpr (!Cfg::ONE_PASS) {
+        float A[Cfg::NANG + 1][Pass1<Cfg>::N];
+        desc_accumulate<Cfg, Pass1<Cfg>, false>(m, g, grad, A);
+        desc_finish<Cfg, Pass1<Cfg>>(A, G);
+    }
+}
+
+// Adjoint pass over range LR, recomputing that range's forward accumulators.
+template <class Cfg, class LR, bool RAD>
+__device__ __forceinline__ void adjoint_pass(const nepx_pair_model& m, const PairIn& g, const float (&dG)[Cfg::D],
+                                             AdjAcc& acc) {
+    float grad[Cfg::NRAD + 1];
+    float A[Cfg::NANG + 1][LR::N];
+    desc_accumulate<Cfg, LR, false>(m, g, grad, A);
+    desc_adjoint<Cfg, LR, RAD>(m, g, dG, A, acc);
+}
+
 // ---------------------------------------------------------------------------
 template <class Cfg>
 __global__ __launch_bounds__(kBlock) void k_desc_fwd(nepx_pair_model m, const float4* __restrict__ qf,
@@ -81,11 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_desc_fwd(nepx_pair_model m, const fl
         float4 qi, qj, t;
         load_pair(qf, pi, pj, pt, idx, qi, qj, t);
         const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
-        float grad[Cfg::NRAD + 1];
-        float A[Cfg::NANG + 1][Cfg::NLM];
-        desc_accumulate<Cfg>(m, g, grad, A);
         float Gr[Cfg::D];
-        desc_finish<Cfg>(grad, A, Gr);
+        desc_forward<Cfg>(m, g, Gr);
         store_row<Cfg>(G + (size_t)idx * Cfg::D, Gr);
     }
 }
@@ -104,13 +129,12 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_mode
         float4 qi, qj, t;
         load_pair(qf, pi, pj, pt, idx, qi, qj, t);
         const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
-        float grad[Cfg::NRAD + 1];
-        float A[Cfg::NANG + 1][Cfg::NLM];
-        desc_accumulate<Cfg>(m, g, grad, A);
         float dG[Cfg::D];
         load_row<Cfg>(dEdG + (size_t)idx * Cfg::D, dG);
-        const PairOut o = desc_adjoint<Cfg>(m, g, dG, A);
-        store_out(out, idx, o, E ? E[idx] : 0.f, clip);
+        AdjAcc acc;
+        adjoint_pass<Cfg, Pass0<Cfg>, true>(m, g, dG, acc);
+        if constexpr (!Cfg::ONE_PASS) adjoint_pass<Cfg, Pass1<Cfg>, false>(m, g, dG, acc);
+        store_out(out, idx, adj_finish<Cfg>(acc), E ? E[idx] : 0.f, clip);
     }
 }
 
@@ -188,8 +212,8 @@ __device__ __forceinline__ void stage_net(float* wl, const float* Wg, const void
     }
 }
 
-template <class N, int NT, bool HALF>
-__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, const void* __restrict__ Whg, int act,
+template <class N, int NT, bool HALF, int ACT>
+__global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, const void* __restrict__ Whg,
                                                  float en_min, float scale, int D, const float* __restrict__ G,
                                                  const int32_t* __restrict__ np_dev, int64_t maxp,
                                                  float* __restrict__ E, float* __restrict__ dEdG) {
@@ -212,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, c
         }
         v4f dg[NT][N::MD];
         float y[NT];
-        R::run(W, WH, act, gin, scale, dg, y);
+        R::run(W, WH, ACT, gin, scale, dg, y);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int p = (tile * NT + t) * 16 + (lane & 15);
@@ -227,10 +251,10 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, c
 // block runs the same number of iterations and the barriers are uniform.
 // The forward accumulators A of each lane's pair stay in registers through
 // the network phase and feed the adjoint directly.
-template <class Cfg, class N, bool HALF>
+template <class Cfg, class N, bool HALF, int ACT>
 __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model m, const float* __restrict__ Wg,
                                                                 const void* __restrict__ Whg,
-                                                                int act, float en_min, float scale,
+                                                                float en_min, float scale,
                                                                 const float4* __restrict__ qf,
                                                                 const int32_t* __restrict__ pi,
                                                                 const int32_t* __restrict__ pj,
@@ -255,12 +279,19 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
         float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
         if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
         const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
-        float A[Cfg::NANG + 1][Cfg::NLM];
+        // one-pass shapes keep the forward accumulators for the adjoint
+        float A[Cfg::ONE_PASS ? Cfg::NANG + 1 : 1][Cfg::ONE_PASS ? Cfg::NLM : 1];
         {
-            float grad[Cfg::NRAD + 1];
-            desc_accumulate<Cfg>(m, g, grad, A);
             float Gr[Cfg::D];
-            desc_finish<Cfg>(grad, A, Gr);
+            if constexpr (Cfg::ONE_PASS) {
+                float grad[Cfg::NRAD + 1];
+                desc_accumulate<Cfg, Pass0<Cfg>, true>(m, g, grad, A);
+                desc_finish<Cfg, Pass0<Cfg>>(A, Gr);
+#pragma unroll
+                for (int n = 0; n <= Cfg::NRAD; ++n) Gr[n] = grad[n];
+            } else {
+                desc_forward<Cfg>(m, g, Gr);
+            }
 #pragma unroll
             for (int k = 0; k < Cfg::D; ++k) myrow[k] = valid ? Gr[k] : 0.f;
         }
@@ -275,7 +306,7 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
             }
             v4f dg[kNetTiles][N::MD];
             float y[kNetTiles];
-            R::run(W, WH, act, gin, scale, dg, y);
+            R::run(W, WH, ACT, gin, scale, dg, y);
 #pragma unroll
             for (int tt = 0; tt < kNetTiles; ++tt) {
                 const int local = wv * 64 + (sub * kNetTiles + tt) * 16 + (lane & 15);
@@ -288,8 +319,14 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
             float dG[Cfg::D];
 #pragma unroll
             for (int k = 0; k < Cfg::D; ++k) dG[k] = myrow[k];
-            const PairOut o = desc_adjoint<Cfg>(m, g, dG, A);
-            store_out(out, idx, o, etile[threadIdx.x], clip);
+            AdjAcc acc;
+            if constexpr (Cfg::ONE_PASS) {
+                desc_adjoint<Cfg, Pass0<Cfg>, true>(m, g, dG, A, acc);
+            } else {
+                adjoint_pass<Cfg, Pass0<Cfg>, true>(m, g, dG, acc);
+                adjoint_pass<Cfg, Pass1<Cfg>, false>(m, g, dG, acc);
+            }
+            store_out(out, idx, adj_finish<Cfg>(acc), etile[threadIdx.x], clip);
         }
         __syncthreads();
     }
@@ -548,6 +585,28 @@ static bool net_supported(const nepx_net* n) {
     return false;
 }
 
+template <class T, int NT, bool HALF_OK>
+static int launch_ffnn(const nepx_net* net, bool half, int grid, float scale, int d, const float* G,
+                       const int32_t* np_dev, int64_t maxp, float* E, float* dEdG, void* stream) {
+    const bool tanh_act = net->act == NEPX_ACT_TANH;
+#define NEPX_LAUNCH_FFNN(H, A)                                                                             \
+    hipLaunchKernelGGL((k_ffnn<T, NT, H, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), net->w_packed, \
+                       net->w_half, net->en_min, scale, d, G, np_dev, maxp, E, dEdG)
+    if constexpr (HALF_OK) {
+        if (half) {
+            if (tanh_act) NEPX_LAUNCH_FFNN(true, NEPX_ACT_TANH);
+            else NEPX_LAUNCH_FFNN(true, NEPX_ACT_RELU);
+            NEPX_LAUNCH_CHECK();
+            return NEPX_OK;
+        }
+    }
+    if (tanh_act) NEPX_LAUNCH_FFNN(false, NEPX_ACT_TANH);
+    else NEPX_LAUNCH_FFNN(false, NEPX_ACT_RELU);
+#undef NEPX_LAUNCH_FFNN
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
 extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int32_t* np_dev, int64_t maxp,
                                  float* E, float* dEdG, void* stream) {
     NEPX_REQUIRE(net && net->w_packed, NEPX_EINVAL, "null network");
@@ -558,24 +617,11 @@ extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int3
     const float scale = net->en_max - net->en_min;
     const bool half = net->precision == NEPX_NET_F16X3;
     NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
-#define X(d, w, h, T, NT)                                                                                    \
-    if (net->d_in == d && net->width == w && net->n_hidden == h) {                                           \
-        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64, 2 * 256);              \
-        if constexpr (T::HALF_OK) {                                                                          \
-            if (half) {                                                                                      \
-                hipLaunchKernelGGL((k_ffnn<T, NT, true>), dim3(grid), dim3(kBlock), 0, as_stream(stream),    \
-                                   net->w_packed, net->w_half, net->act, net->en_min, scale, d, G, np_dev,   \
-                                   maxp, E, dEdG);                                                           \
-                NEPX_LAUNCH_CHECK();                                                                         \
-                return NEPX_OK;                                                                              \
-            }                                                                                                \
-        }                                                                                                    \
-        NEPX_REQUIRE(!half, NEPX_EUNSUP, "no fp16 network kernel for width %d", w);                          \
-        hipLaunchKernelGGL((k_ffnn<T, NT, false>), dim3(grid), dim3(kBlock), 0, as_stream(stream),           \
-                           net->w_packed, net->w_half, net->act, net->en_min, scale, d, G, np_dev, maxp, E,  \
-                           dEdG);                                                                            \
-        NEPX_LAUNCH_CHECK();                                                                                 \
-        return NEPX_OK;                                                                                      \
+#define X(d, w, h, T, NT)                                                                        \
+    if (net->d_in == d && net->width == w && net->n_hidden == h) {                               \
+        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64, 2 * 256);  \
+        if constexpr (!T::HALF_OK) NEPX_REQUIRE(!half, NEPX_EUNSUP, "no fp16 network kernel for width %d", w); \
+        return launch_ffnn<T, NT, T::HALF_OK>(net, half, grid, scale, d, G, np_dev, maxp, E, dEdG, stream); \
     }
     NEPX_NET_CASES(X)
 #undef X
@@ -620,6 +666,27 @@ extern "C" int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_ne
     return 0;
 }
 
+template <class Cfg, class N>
+static int launch_fused(const nepx_pair_model& m, const nepx_net* net, bool half, int grid, float scale,
+                        const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                        const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+    const bool tanh_act = net->act == NEPX_ACT_TANH;
+#define NEPX_LAUNCH_FUSED(H, A)                                                                                  \
+    hipLaunchKernelGGL((k_fused<Cfg, N, H, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), m, net->w_packed, \
+                       net->w_half, net->en_min, scale, (const float4*)qf, pair_i, pair_j, (const float4*)pair_t,  \
+                       np_dev, maxp, clip, out)
+    if (half) {
+        if (tanh_act) NEPX_LAUNCH_FUSED(true, NEPX_ACT_TANH);
+        else NEPX_LAUNCH_FUSED(true, NEPX_ACT_RELU);
+    } else {
+        if (tanh_act) NEPX_LAUNCH_FUSED(false, NEPX_ACT_TANH);
+        else NEPX_LAUNCH_FUSED(false, NEPX_ACT_RELU);
+    }
+#undef NEPX_LAUNCH_FUSED
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
 extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
                                const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
                                const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
@@ -633,21 +700,11 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     const int grid = grid_for_cap(maxp, kBlock, 256);
     const bool half = net->precision == NEPX_NET_F16X3;
     NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
-#define X(ns, a, b, c, T, d, w, h, NT)                                                                      \
-    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&               \
-        net->width == w && net->n_hidden == h) {                                                            \
-        if (half) {                                                                                         \
-            hipLaunchKernelGGL((k_fused<T, NT, true>), dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,  \
-                               net->w_packed, net->w_half, net->act, net->en_min, scale, (const float4*)qf, \
-                               pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, clip, out);             \
-        } else {                                                                                            \
-            hipLaunchKernelGGL((k_fused<T, NT, false>), dim3(grid), dim3(kBlock), 0, as_stream(stream), *m, \
-                               net->w_packed, net->w_half, net->act, net->en_min, scale, (const float4*)qf, \
-                               pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, clip, out);             \
-        }                                                                                                   \
-        NEPX_LAUNCH_CHECK();                                                                                \
-        return NEPX_OK;                                                                                     \
-    }
+#define X(ns, a, b, c, T, d, w, h, NT)                                                               \
+    if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&        \
+        net->width == w && net->n_hidden == h)                                                       \
+        return launch_fused<T, NT>(*m, net, half, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip, \
+                                   out, stream);
     NEPX_FUSED_CASES(X)
 #undef X
     set_error("no fused kernel for n_sites=%d hypers=(%d,%d,%d) net %d->%dx%d", m->n_sites, m->nrad,
