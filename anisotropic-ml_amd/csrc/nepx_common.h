@@ -34,6 +34,10 @@ void set_error(const char* fmt, ...);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// ---- launch tuning (host) ----------------------------------------------------
+nepx_tuning tuning();        // current values with defaults filled in
+int device_cus();            // compute units of the current device
+
 // ---- compile-time helpers --------------------------------------------------
 constexpr double csqrt(double x) {
     double r = x > 1.0 ? x : 1.0;

@@ -86,7 +86,11 @@ __device__ __forceinline__ double disp2(const nepx_state& st, int i, double L) {
 // 5 = kinetic energies + displacement for the thermostat tail.
 enum { M_STEP = 0, M_KE = 1, M_TWO = 2, M_TRANS = 3, M_ROT = 4, M_THERMO = 5 };
 
-__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode) {
+// With `flags` set and the list overflowed, the step is skipped as a whole
+// (uniformly: every thread reads the same flag) so the state stays at the
+// last good step until the host has grown the list.
+__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, const int32_t* __restrict__ flags) {
+    if (flags && flags[1]) return;
     const int i = blockIdx.x * kIB + threadIdx.x;
     double kt = 0.0, kr = 0.0, u = 0.0, d2 = 0.0;
     if (i < st.n && st.sys[i] >= 0 && st.sp[st.sys[i]].active) {
@@ -183,6 +187,7 @@ __global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* f
     const int s = blockIdx.x;
     nepx_sysparam& sp = st.sp[s];
     if (!sp.active) return;  // uniform per block
+    if (flags && flags[1]) return;  // list overflow: the step was skipped
     if (update == 2 && st.th[s].pending == 0.0) return;  // only newly admitted systems
     double a = 0, b = 0, c = 0, d = 0;
     for (int k = sp.blk_start + threadIdx.x; k < sp.blk_end; k += kIB) {
@@ -248,7 +253,7 @@ extern "C" int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream) {
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0, nl ? (const int32_t*)nl->flags : nullptr);
     hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
@@ -259,7 +264,7 @@ extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_ther
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1, (const int32_t*)nullptr);
     hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr,
                        update_thermostat == 2 ? 2 : (update_thermostat ? 1 : 0), 0);
     NEPX_LAUNCH_CHECK();
@@ -274,7 +279,7 @@ extern "C" int nepx_nvt_part(nepx_state* st, nepx_nlist* nl, int32_t part, void*
     NEPX_REQUIRE(st->dp, NEPX_EINVAL, "null dp buffer");
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part, nl ? (const int32_t*)nl->flags : nullptr);
     if (part == NEPX_NVT_THERMO)
         hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();

@@ -403,24 +403,25 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
 }
 
 // ---- gather ----------------------------------------------------------------
-// 16 lanes per particle: lanes stride over the particle's forward row (i-role:
-// +F, +tau_i, +E) and reverse row (j-role: -F, +tau_j), accumulate in FP64 and
-// reduce with a fixed butterfly, so the result is independent of timing.
-constexpr int kGL = 16;
-
-__device__ __forceinline__ double red16(double v) {
+// GL lanes per particle (tuned; 16 by default): lanes stride over the
+// particle's forward row (i-role: +F, +tau_i, +E) and reverse row (j-role:
+// -F, +tau_j), accumulate in FP64 and reduce with a fixed butterfly, so the
+// result is independent of timing (and, for one GL, of the launch).
+template <int GL>
+__device__ __forceinline__ double red_lanes(double v) {
 #pragma unroll
-    for (int o = kGL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kGL);
+    for (int o = GL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, GL);
     return v;
 }
 
+template <int GL>
 __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, const float4* __restrict__ out) {
-    const int k = (blockIdx.x * kNB + threadIdx.x) / kGL;
-    const int sub = threadIdx.x % kGL;
-    if (k >= st.n) return;  // whole 16-lane groups exit together
+    const int k = (blockIdx.x * kNB + threadIdx.x) / GL;
+    const int sub = threadIdx.x % GL;
+    if (k >= st.n) return;  // whole lane groups exit together
     double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, u = 0;
     if (live(st, k) && !nl.flags[F_OVERFLOW]) {
-        for (int s = nl.row_start[k] + sub; s < nl.row_start[k + 1]; s += kGL) {
+        for (int s = nl.row_start[k] + sub; s < nl.row_start[k + 1]; s += GL) {
             const int c = nl.cidx[s];
             if (c < 0) continue;
             const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 1];
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
             ty += b.y;
             tz += b.z;
         }
-        for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += kGL) {
+        for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += GL) {
             const int c = nl.cidx[nl.rnbr[e]];
             if (c < 0) continue;
             const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 2];
@@ -444,13 +445,13 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
             tz += b.z;
         }
     }
-    fx = red16(fx);
-    fy = red16(fy);
-    fz = red16(fz);
-    tx = red16(tx);
-    ty = red16(ty);
-    tz = red16(tz);
-    u = red16(u);
+    fx = red_lanes<GL>(fx);
+    fy = red_lanes<GL>(fy);
+    fz = red_lanes<GL>(fz);
+    tx = red_lanes<GL>(tx);
+    ty = red_lanes<GL>(ty);
+    tz = red_lanes<GL>(tz);
+    u = red_lanes<GL>(u);
     if (sub == 0) {
         st.fx[k] = fx;
         st.fy[k] = fy;
@@ -572,8 +573,18 @@ extern "C" int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* p
     if (rc) return rc;
     NEPX_REQUIRE(pair_out && st->fx && st->fy && st->fz && st->tx && st->ty && st->tz && st->upot, NEPX_EINVAL,
                  "null force buffer");
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for((int64_t)st->n * kGL, kNB)), dim3(kNB), 0, as_stream(stream), *st, *nl,
-                       (const float4*)pair_out);
+    const int gl = tuning().gather_lanes;
+#define NEPX_GATHER(GL)                                                                                   \
+    hipLaunchKernelGGL(k_gather<GL>, dim3(blocks_for((int64_t)st->n * GL, kNB)), dim3(kNB), 0,             \
+                       as_stream(stream), *st, *nl, (const float4*)pair_out)
+    switch (gl) {
+        case 4: NEPX_GATHER(4); break;
+        case 8: NEPX_GATHER(8); break;
+        case 32: NEPX_GATHER(32); break;
+        case 64: NEPX_GATHER(64); break;
+        default: NEPX_GATHER(16); break;
+    }
+#undef NEPX_GATHER
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }

@@ -412,11 +412,7 @@ static int grid_for_cap(int64_t units, int per_block, int64_t cap) {
 }
 
 static int grid_for(int64_t maxp, int per_block) {
-    int64_t g = (maxp + per_block - 1) / per_block;
-    const int64_t cap = 256 * 16;  // persistent cap: 16 blocks per CU on 256 CUs
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    return (int)g;
+    return grid_for_cap(maxp, per_block, (int64_t)device_cus() * tuning().pair_blocks_per_cu);
 }
 
 #define NEPX_DESC_CASES(X)                          \
@@ -619,7 +615,8 @@ extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int3
     NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
 #define X(d, w, h, T, NT)                                                                        \
     if (net->d_in == d && net->width == w && net->n_hidden == h) {                               \
-        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64, 2 * 256);  \
+        const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64,            \
+                                      (int64_t)device_cus() * tuning().net_blocks_per_cu);       \
         if constexpr (!T::HALF_OK) NEPX_REQUIRE(!half, NEPX_EUNSUP, "no fp16 network kernel for width %d", w); \
         return launch_ffnn<T, NT, T::HALF_OK>(net, half, grid, scale, d, G, np_dev, maxp, E, dEdG, stream); \
     }
@@ -697,7 +694,7 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && out, NEPX_EINVAL, "null buffer");
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-    const int grid = grid_for_cap(maxp, kBlock, 256);
+    const int grid = grid_for_cap(maxp, kBlock, (int64_t)device_cus() * tuning().fused_blocks_per_cu);
     const bool half = net->precision == NEPX_NET_F16X3;
     NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
 #define X(ns, a, b, c, T, d, w, h, NT)                                                               \

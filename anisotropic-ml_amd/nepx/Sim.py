@@ -238,20 +238,28 @@ class Sim:
         self.timer = time.time()
         self.epoch_timer = time.time()
         check_every = 100
+        base = self.arena.device_steps()
         while self.timestep < Nsteps:
             if self.gsdDumpPeriod and self.gsdFilename and self.timestep % self.gsdDumpPeriod == 0:
                 torch.cuda.synchronize(self.device)
                 self.print_performance_info()
                 self.dumpConfig(self.timestep)
             self.step()
-            if self.timestep % check_every == 0:
-                self.arena.check()
+            if self.timestep % check_every == 0 or self.timestep == Nsteps:
+                self._recover(base)
         torch.cuda.synchronize(self.device)
         self.arena.check()
         print("Timers:")
         for k in ("t_nlist", "t_gen", "t_eval", "t_s1", "t_s2"):
             print("%s : %.2f" % (k, getattr(self, k)))
         print("Done")
+
+    def _recover(self, base):
+        """A list overflow makes the device skip steps; grow the list and resume from the device count."""
+        handled = self.runner.recover() if self.runner is not None else self.arena.recover_overflow()
+        if handled:
+            self.timestep = self.arena.device_steps() - base
+        self.arena.check()
 
     def step(self):
         self.descriptor_generator.fd_compat = self.fd_compat

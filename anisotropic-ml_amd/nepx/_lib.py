@@ -51,6 +51,11 @@ class NList(C.Structure):
     _fields_ = [("cap", C.c_int64), ("n_cells_total", C.c_int32)] + [(k, C.c_void_p) for k in _NL_PTRS]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("fused_blocks_per_cu", C.c_int32), ("pair_blocks_per_cu", C.c_int32),
+                ("gather_lanes", C.c_int32), ("net_blocks_per_cu", C.c_int32)]
+
+
 # numpy mirrors of the per-system records
 SYSPARAM_DTYPE = np.dtype([("L", "f8"), ("kT", "f8"), ("dt", "f8"), ("tau", "f8"), ("ndof", "f8"),
                            ("rndof", "f8"), ("moi", "f8", 3), ("r_list", "f8"), ("rc", "f8"),
@@ -80,6 +85,8 @@ def _load():
         "nepx_last_error": ([], C.c_char_p),
         "nepx_version": ([], C.c_int),
         "nepx_struct_sizes": ([c_i32p], C.c_int),
+        "nepx_set_tuning": ([C.POINTER(Tuning)], C.c_int),
+        "nepx_get_tuning": ([C.POINTER(Tuning)], C.c_int),
         "nepx_net_packed_floats": ([C.c_int32, C.c_int32, C.c_int32], C.c_int64),
         "nepx_net_pack": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(c_f32p), C.POINTER(c_f32p), c_f32p],
                           C.c_int),
@@ -111,9 +118,9 @@ def _load():
     sizes = (C.c_int32 * 8)()
     lib.nepx_struct_sizes(sizes)
     expect = [C.sizeof(PairModel), C.sizeof(Net), SYSPARAM_DTYPE.itemsize, THERMO_DTYPE.itemsize,
-              C.sizeof(State), C.sizeof(NList)]
-    if list(sizes[:6]) != expect:
-        raise NepxError(f"struct layout mismatch: library {list(sizes[:6])} vs binding {expect}")
+              C.sizeof(State), C.sizeof(NList), C.sizeof(Tuning)]
+    if list(sizes[:7]) != expect:
+        raise NepxError(f"struct layout mismatch: library {list(sizes[:7])} vs binding {expect}")
     return lib
 
 

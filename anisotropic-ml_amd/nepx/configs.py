@@ -118,6 +118,22 @@ def make_system(shape_name, n, phi, kT=1.0, dt=0.001, seed=0, rc=None, skin=0.5,
                       thermo=thermo)
 
 
+def replica_specs(n_systems, seed, shape_name="cube8", n_range=(64, 4096), phi_range=(0.2, 0.5),
+                  kT_range=(0.5, 2.0)):
+    """Independent systems for batched runs: N log-uniform, packing and kT uniform (seeded).
+    Positions are a jittered lattice with random orientations (bodies may overlap); the packing
+    fraction is capped so that every box is at least 2.05 cutoffs wide (minimum image)."""
+    shape = get_shape(shape_name)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_systems):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        phi = min(float(rng.uniform(*phi_range)), n * shape.volume / (2.05 * shape.cutoff) ** 3)
+        kT = float(rng.uniform(*kT_range))
+        out.append(make_system(shape_name, n, phi, kT=kT, seed=int(rng.integers(1 << 30)), non_overlap=False))
+    return out
+
+
 # Named workloads (sizes as in the benchmark definitions).
 WORKLOADS = {
     "cube512": dict(shape="cube8", n=512, phi=0.30, kT=1.0, dt=0.001),

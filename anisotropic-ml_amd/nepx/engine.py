@@ -85,6 +85,16 @@ class StepRunner:
         for _ in range(n):
             self.step()
 
+    def recover(self):
+        """Handle a list overflow seen at a synchronisation point: grow and rebuild the list and
+        recapture the step.  Steps launched while the list was overflowed did nothing; callers
+        count progress with Arena.device_steps().  Returns True when an overflow was handled."""
+        if not self.ar.recover_overflow(self.stream):
+            return False
+        if self.use_graph:
+            self._capture()
+        return True
+
     # timing -----------------------------------------------------------------
     def enable_kernel_events(self, n):
         self._ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]

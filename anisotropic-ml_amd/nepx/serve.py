@@ -265,6 +265,7 @@ class Server:
             ar = SlotArena(shape, hyp, net, capacity, max_systems, device=self.device)
             self.classes[name] = _ClassRunner(ar)
         self.chunk = chunk
+        self.recoveries = 0
 
     def run(self, jobs, time_limit=None, log=None):
         queue = collections.deque(sorted(jobs, key=lambda j: j.arrival))
@@ -306,6 +307,9 @@ class Server:
             evs = [(cr, cr.run_chunk(self.chunk)) for cr in busy]
             for cr, ev in evs:
                 cr.runner.stream.synchronize()
+                if cr.runner.recover():     # list overflow: those steps were skipped on the device
+                    self.recoveries += 1
+                    continue
                 lat.extend(ev[i].elapsed_time(ev[i + 1]) for i in range(len(ev) - 1))
                 with torch.cuda.stream(cr.runner.stream):
                     retired = cr.arena.harvest()
@@ -319,12 +323,21 @@ class Server:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         resident_ps = sum(cr.arena.particle_steps() for cr in self.classes.values())
+        resident = []
+        for cr in self.classes.values():
+            if cr.arena.jobs:
+                _, th = cr.arena.records()
+                for slot, j in cr.arena.jobs.items():
+                    j.steps_done = int(th[slot]["steps"])
+                    resident.append(j)
+        self.last_resident = resident
         for cr in self.classes.values():
             cr.arena.check()
         self.last_finished = finished
         self.last_rejected = rejected
         st = _stats("served", finished, retired_ps + resident_ps, wall, lat)
         st["jobs_rejected"] = len(rejected)
+        st["list_regrowths"] = self.recoveries
         return st
 
 
@@ -344,6 +357,7 @@ def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None)
     done_ps = 0
     finished = []
     lat = []
+    regrowths = 0
     for j in sorted(jobs, key=lambda j: j.arrival):
         now = time.perf_counter() - t0
         if time_limit is not None and now >= time_limit:
@@ -366,7 +380,11 @@ def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None)
                 runner.step()
                 evs[i + 1].record(st)
             st.synchronize()
-            lat.extend(evs[i].elapsed_time(evs[i + 1]) for i in range(k))
+            if runner.recover():          # list overflow: the device skipped steps; count what it did
+                regrowths += 1
+                k = ar.device_steps() - (j.steps - left)
+            else:
+                lat.extend(evs[i].elapsed_time(evs[i + 1]) for i in range(k))
             left -= k
             done_ps += j.n * k
             if log and time.perf_counter() - t0 - last_log > 5.0:
@@ -382,7 +400,50 @@ def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None)
             finished.append(j)
         ar.check()
     torch.cuda.synchronize()
-    return _stats("plain", finished, done_ps, time.perf_counter() - t0, lat)
+    st = _stats("plain", finished, done_ps, time.perf_counter() - t0, lat)
+    st["list_regrowths"] = regrowths
+    return st
+
+
+def plain_step_costs(jobs, classes, device="cuda", steps=20, warmup=5, log=None):
+    """Per job: ms per step of the plain single-system captured step on that job's system
+    (same N, box, kT), timed with hipEvents after `warmup` steps on a copy of the system."""
+    import copy
+    dev = torch.device(device)
+    costs = {}
+    t0 = time.perf_counter()
+    for k, j in enumerate(jobs):
+        shape, hyp, net = classes[j.cls]
+        ar = Arena([copy.deepcopy(j.spec)], shape, hyp, net, device=dev)
+        r = StepRunner(ar, use_graph=True)
+        r.init()
+        r.run(warmup)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(r.stream)
+        r.run(steps)
+        b.record(r.stream)
+        r.stream.synchronize()
+        if r.recover():
+            a.record(r.stream)
+            r.run(steps)
+            b.record(r.stream)
+            r.stream.synchronize()
+        ar.check()
+        costs[j.jid] = a.elapsed_time(b) / steps
+        del r, ar
+        if log and (k + 1) % 16 == 0:
+            log(time.perf_counter() - t0, {"plain_costs": k + 1}, len(jobs))
+    return costs
+
+
+def same_work_plain(jobs, costs, sample_scale=1.0):
+    """Time the plain one-system-at-a-time step needs for the work the jobs record in steps_done:
+    sum_j steps_done_j * ms_per_step_j (over the jobs that have a cost, times sample_scale)."""
+    ms = sum(j.steps_done * costs[j.jid] for j in jobs if j.jid in costs) * sample_scale
+    work = sum(j.n * j.steps_done for j in jobs)
+    return {"particle_steps": int(work), "seconds": ms * 1e-3,
+            "particle_steps_per_s": work / (ms * 1e-3) if ms > 0 else float("nan"),
+            "jobs": len(jobs), "jobs_timed": sum(1 for j in jobs if j.jid in costs)}
 
 
 def poisson_jobs(rate, duration, seed, shapes=("cube8", "tet10"), n_range=(128, 65536),

@@ -219,6 +219,28 @@ class Arena:
         self.nvt_step(stream)
         self.steps += 1
 
+    def recover_overflow(self, stream=None):
+        """Synchronising: if the list overflowed (the integrator then skips its steps, so the state
+        is still that of the last good step), grow the list and rebuild it.  Returns True when an
+        overflow was handled; the caller recaptures any graph of the step (the buffers moved)."""
+        if stream is not None:
+            stream.synchronize()
+        f = self.flags()
+        code = int(f[_lib.FLAG_OVERFLOW])
+        if code == 0:
+            return False
+        if code != 1:
+            raise _lib.NepxError(f"inconsistent neighbour list (code {code}, flags {f.tolist()})")
+        need = max(int(f[_lib.FLAG_PL]), self.cap)
+        self._alloc_list(int(1.5 * need) + 4096)
+        self.build_list(True)
+        self.ensure_capacity()
+        return True
+
+    def device_steps(self, k=0):
+        """Steps system k has completed on the device (skipped steps are not counted)."""
+        return int(self.thermo()[k]["steps"])
+
     def check(self):
         f = self.flags()
         if f[_lib.FLAG_OVERFLOW]:

@@ -82,32 +82,108 @@ def serve_bench(a):
     import copy
     import torch
     import nepx
-    from nepx.serve import Server, poisson_jobs, run_plain
+    import numpy as np
+    from nepx.serve import Server, plain_step_costs, poisson_jobs, run_plain, same_work_plain
     dev = torch.device("cuda:0")
-    net = nepx.EnergyNet.random(40, [64, 64], seed=7, device=dev)
-    classes = {name: (nepx.get_shape(name), (11, 6, 4), net) for name in ("cube8", "tet10")}
-    jobs = poisson_jobs(rate=20.0, duration=a.serve_arrivals, seed=2025, shapes=("cube8", "tet10"))
+    nets = {40: nepx.EnergyNet.random(40, [64, 64], seed=7, device=dev, precision=a.precision),
+            60: nepx.EnergyNet.random(60, [128, 128], seed=7, device=dev, precision=a.precision)}
+    classes = {}
+    for name in a.serve_shapes.split(","):
+        sh = nepx.get_shape(name)
+        classes[name] = (sh, sh.hypers, nets[nepx.n_descriptor(*sh.hypers)])
+    jobs = poisson_jobs(rate=20.0, duration=a.serve_arrivals, seed=2025, shapes=tuple(classes))
     plain_jobs = copy.deepcopy(jobs)
     def log(t, resident, done):
         print(f"[serve] t={t:.0f}s resident={resident} finished={done}", file=sys.stderr, flush=True)
     srv = Server(classes, capacity=a.serve_capacity, max_systems=1024, chunk=a.serve_chunk, device=dev)
     served = srv.run(jobs, time_limit=a.serve_seconds, log=log)
-    plain = run_plain(plain_jobs, classes, time_limit=a.serve_seconds, chunk=a.serve_chunk, device=dev, log=log)
-    return {
-        "metric": "served-batch vs plain particle-steps/s (mixed cube/tet job trace); p99 step latency",
+    # plain on the same work: every job the server worked on, timed alone with the single-system step
+    worked = [j for j in srv.last_finished + srv.last_resident if j.steps_done > 0]
+    rng = np.random.default_rng(0)
+    timed = (worked if len(worked) <= a.serve_plain_jobs else
+             [worked[i] for i in rng.choice(len(worked), a.serve_plain_jobs, replace=False)])
+    costs = plain_step_costs(timed, classes, dev, log=log)
+    # uniform sample: scale the sampled plain time up to all jobs
+    plain_same = same_work_plain(worked, costs, sample_scale=len(worked) / max(len(timed), 1))
+    res = {
+        "metric": "served-batch vs plain particle-steps/s (mixed cube/tet/cylinder job trace); p99 step latency",
         "value": served["particle_steps_per_s"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "higher_is_better": True,
-        "served_over_plain": served["particle_steps_per_s"] / plain["particle_steps_per_s"],
+        "served_over_plain": served["particle_steps_per_s"] / plain_same["particle_steps_per_s"],
         "served": served,
-        "plain": plain,
+        "plain_same_work": plain_same,
         "config": {"workload": f"Poisson 20 jobs/s for {a.serve_arrivals} s of arrivals ({len(jobs)} jobs), "
-                               "cube8/tet10, N log-uniform [128, 65536], steps log-uniform [1e3, 1e5], "
-                               "kT U[0.5, 2], phi U[0.2, 0.45], FFNN 40->64->64->1 tanh",
-                   "window_s": a.serve_seconds, "capacity_per_class": a.serve_capacity, "chunk": a.serve_chunk},
-        "dtype": "f32 pair math (MFMA f32), f64 integrator",
+                               f"{'/'.join(classes)}, N log-uniform [128, 65536], steps log-uniform [1e3, 1e5], "
+                               "kT U[0.5, 2], phi U[0.2, 0.45], FFNN D->64->64->1 (D=40) or D->128->128->1 "
+                               "(D=60) tanh",
+                   "window_s": a.serve_seconds, "capacity_per_class": a.serve_capacity, "chunk": a.serve_chunk,
+                   "jobs_per_class": {c: sum(j.cls == c for j in jobs) for c in classes}},
+        "dtype": f"f32 pair math ({'fp16 hi/lo split MFMA' if nets[40].precision == 'f16x3' else 'MFMA f32'}), "
+                 "f64 integrator",
         "data": "synthetic",
+    }
+    if a.serve_plain_window:
+        res["plain_window"] = run_plain(plain_jobs, classes, time_limit=a.serve_seconds, chunk=a.serve_chunk,
+                                        device=dev, log=log)
+    return res
+
+
+def replicas_bench(a):
+    """256 independent cube systems advanced by the same launches (one arena, hipGraph steps) against
+    the same systems run one at a time with the single-system step."""
+    import torch
+    import nepx
+    from nepx.configs import replica_specs
+    from nepx.engine import StepRunner
+    from nepx.serve import Job, plain_step_costs, same_work_plain
+    dev = torch.device("cuda:0")
+    shape = nepx.get_shape("cube8")
+    hyp = shape.hypers
+    net = nepx.EnergyNet.random(40, [64, 64], seed=7, device=dev, precision=a.precision)
+    specs = replica_specs(256, seed=2026)
+    tuning = None
+    if a.autotune:
+        from nepx.tune import Autotuner
+        tr = Autotuner().tune(specs, shape, hyp, net, dev, steps=20, warmup=5, use_cache=a.autotune == 1)
+        for sp in specs:
+            sp.skin = tr.best.skin
+        tuning = tr.summary()
+    ar = nepx.Arena(specs, shape, hyp, net, device=dev)
+    r = StepRunner(ar, use_graph=True)
+    r.init()
+    r.run(a.warmup)
+    r.stream.synchronize()
+    ar.check()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(r.stream)
+    r.run(a.steps)
+    t1.record(r.stream)
+    r.stream.synchronize()
+    ar.check()
+    ms = t0.elapsed_time(t1) / a.steps
+    n_tot = sum(sp.n for sp in specs)
+    jobs = [Job(k, "cube8", sp, a.steps) for k, sp in enumerate(specs)]
+    for j in jobs:
+        j.steps_done = a.steps
+    log = lambda t, what, n: print(f"[replicas] t={t:.0f}s {what} of {n}", file=sys.stderr, flush=True)
+    costs = plain_step_costs(jobs, {"cube8": (shape, hyp, net)}, dev, log=log)
+    plain = same_work_plain(jobs, costs)
+    return {
+        "metric": "batched replicas vs plain particle-steps/s (256 cube systems)",
+        "value": n_tot / (ms * 1e-3), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "batched_over_plain": (n_tot / (ms * 1e-3)) / plain["particle_steps_per_s"],
+        "plain_same_work": plain,
+        "plain_ms_per_step": {"min": min(costs.values()), "max": max(costs.values()),
+                              "sum_over_systems": sum(costs.values())},
+        "config": {"workload": "256 cube8 systems, N log-uniform [64, 4096], phi U[0.2, 0.5], kT U[0.5, 2], "
+                               "rc 2.5, hypers (11, 6, 4), FFNN 40->64->64->1 tanh, hipGraph steps",
+                   "particles": n_tot, "systems": len(specs)},
+        "dtype": f"f32 pair math ({'fp16 hi/lo split MFMA' if net.precision == 'f16x3' else 'MFMA f32'}), "
+                 "f64 integrator",
+        "data": "synthetic", "tuning": tuning,
     }
 
 
@@ -117,6 +193,9 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="tet32k")
+    ap.add_argument("--autotune", type=int, default=1, choices=[0, 1, 2],
+                    help="pick skin and launch shapes by timing candidates before the timed region "
+                         "(1: use the cache when it has this workload, 2: always re-measure, 0: defaults)")
     ap.add_argument("--precision", default="auto", choices=["auto", "f32", "f16x3"],
                     help="network MFMA operands: FP32, or fp16 hi/lo split (3 products, FP32 accumulate)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -128,6 +207,11 @@ def main():
     ap.add_argument("--serve-arrivals", type=float, default=20.0, help="serve workload: seconds of Poisson arrivals")
     ap.add_argument("--serve-capacity", type=int, default=262144, help="serve workload: particles per class arena")
     ap.add_argument("--serve-chunk", type=int, default=20, help="serve workload: steps per scheduling chunk")
+    ap.add_argument("--serve-shapes", default="cube8,tet10,cyl12", help="serve workload: job classes")
+    ap.add_argument("--serve-plain-jobs", type=int, default=96,
+                    help="serve workload: jobs timed alone for the same-work plain comparison (uniform sample)")
+    ap.add_argument("--serve-plain-window", type=int, default=0,
+                    help="serve workload: also run the jobs one at a time for the same wall window")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "nepx runs on one GPU per process; multi-GPU runs are not part of this build "
@@ -136,6 +220,9 @@ def main():
 
     if a.workload == "serve":
         print(json.dumps(serve_bench(a)))
+        return
+    if a.workload == "replicas256":
+        print(json.dumps(replicas_bench(a)))
         return
 
     import numpy as np
@@ -153,6 +240,13 @@ def main():
     spec = make_system(w["shape"], w["n"], w["phi"], kT=w["kT"], dt=w["dt"], seed=2024)
     dev = torch.device("cuda:0")
     net = nepx.EnergyNet.random(D, widths, seed=7, device=dev, precision=a.precision)
+    tuning = None
+    if a.autotune:
+        from nepx.tune import Autotuner
+        tr = Autotuner().tune([spec], shape, hyp, net, dev, steps=40, warmup=10, use_cache=a.autotune == 1)
+        spec.skin = tr.best.skin
+        tuning = tr.summary()
+        print(f"[bench] autotune {tr.seconds:.1f} s: {tuning['best']}", file=sys.stderr, flush=True)
     ar = nepx.Arena([spec], shape, hyp, net, device=dev)
     runner = StepRunner(ar, use_graph=bool(a.graph))
     runner.init()
@@ -186,7 +280,8 @@ def main():
     mfma_desc = "fp16 hi/lo split MFMA (3 products per FP32 product, FP32 accumulate)" if half else "MFMA f32"
     traffic, traffic_src = profiled_traffic(a.workload)
     res = {
-        "metric": "particle-steps/s at 32k tetrahedra",
+        "metric": ("particle-steps/s at 32k tetrahedra" if a.workload == "tet32k"
+                   else f"particle-steps/s ({a.workload})"),
         "value": spec.n / (ms_step * 1e-3),
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -198,7 +293,7 @@ def main():
         "vs_baseline": None,
         "dtype": f"f32 pair math ({mfma_desc}), f64 integrator",
         "data": "synthetic (seeded lattice + random orientations, random-init FFNN)",
-        "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc=2.5 hypers={hyp} D={D} "
+        "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc={shape.cutoff} hypers={hyp} D={D} "
                                f"FFNN {D}->{widths[0]}->{widths[1]}->1 tanh, NVT kT={w['kT']} dt={w['dt']}",
                    "particles": spec.n, "pairs_per_step": P, "hipgraph": bool(a.graph),
                    "list_builds": int(flags[4]), "wall_s": wall},
@@ -207,6 +302,7 @@ def main():
                      "kernel": "k_fused (descriptor + FFNN fwd/bwd + adjoint)", "kernel_ms": fused_ms,
                      "flops_per_launch": flops},
         "thermo": {"Kt_per_dof": float(th["Kt"] / spec.ndof * 2), "U": float(th["U"])},
+        "tuning": tuning,
     }
     if a.copy_calib:
         n = a.copy_calib * (1 << 20) // 4

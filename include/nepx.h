@@ -124,7 +124,22 @@ typedef struct nepx_nlist {
 const char* nepx_last_error(void);
 int nepx_version(void);
 /* Sizes of the shared structs, for binding checks from the host language. */
-int nepx_struct_sizes(int32_t* out4);
+int nepx_struct_sizes(int32_t* out7);
+
+/* ---- launch tuning ------------------------------------------------------ */
+/* Launch parameters chosen by the autotuner (nepx/tune.py).  They are
+ * process-wide and read at launch time, so a captured graph keeps the values
+ * that were in force when it was captured.  Zero fields select the defaults.
+ * No counterpart in the reference (its launch shapes are fixed in
+ * nep_simulator/NVT.py:46-111 and DescriptorGenerator.py:349-397). */
+typedef struct nepx_tuning {
+    int32_t fused_blocks_per_cu;   /* grid of the fused pair kernel = CUs x this (1..8; default 1)      */
+    int32_t pair_blocks_per_cu;    /* grid cap of the descriptor / adjoint kernels (1..64; default 16)  */
+    int32_t gather_lanes;          /* lanes per particle in the force gather: 4..64, power of 2 (16)   */
+    int32_t net_blocks_per_cu;     /* grid cap of the split network kernel (1..16; default 2)          */
+} nepx_tuning;
+int nepx_set_tuning(const nepx_tuning* t);
+int nepx_get_tuning(nepx_tuning* t);
 
 /* ---- network ----------------------------------------------------------- */
 /* Bytes of the packed weight buffer for these widths. */
@@ -198,7 +213,9 @@ int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* pair_out, vo
 /* ---- integrator -------------------------------------------------------- */
 /* One NVT step given F, T: integrate_step_two + integrate_step_one
  * (NVT.py:17-129) in FP64, then per-system kinetic energies, thermostat
- * update for the next step (update_variables) and the list rebuild test. */
+ * update for the next step (update_variables) and the list rebuild test.
+ * While nl->flags reports a list overflow the step does nothing, so the state
+ * stays at the last good step until the caller has grown the list. */
 int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream);
 /* Kinetic energies of the current state and, when update_thermostat != 0, the
  * thermostat update that the reference performs at the start of the first step

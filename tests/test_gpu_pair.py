@@ -100,7 +100,8 @@ def test_adjoint_matches_oracle(gpu, shape_name, hyp):
 
 
 @pytest.mark.parametrize("precision", ["f32", "f16x3"])
-@pytest.mark.parametrize("shape_name,hyp,widths", [("tet10", (11, 6, 4), [64, 64]), ("cube8", (11, 6, 4), [64, 64])])
+@pytest.mark.parametrize("shape_name,hyp,widths", [("tet10", (11, 6, 4), [64, 64]), ("cube8", (11, 6, 4), [64, 64]),
+                                                   ("cyl12", (17, 6, 6), [128, 128])])
 def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision):
     import nepx
     from nepx import ops

@@ -44,9 +44,11 @@ def test_pair_set_bit_exact(gpu, shape_name, n, phi):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("precision", ["f32", "f16x3"])
-def test_system_forces_match_oracle(gpu, precision):
-    ar, spec, net = _arena(gpu, "tet10", 400, 0.1, seed=4, precision=precision)
+@pytest.mark.parametrize("shape_name,n,hyp,widths,precision",
+                         [("tet10", 400, (11, 6, 4), (64, 64), "f32"), ("tet10", 400, (11, 6, 4), (64, 64), "f16x3"),
+                          ("cyl12", 200, (17, 6, 6), (128, 128), "f16x3")])
+def test_system_forces_match_oracle(gpu, shape_name, n, hyp, widths, precision):
+    ar, spec, net = _arena(gpu, shape_name, n, 0.1, seed=4, widths=widths, hyp=hyp, precision=precision)
     ar.init()
     ar.filter_pairs()
     ar.pair_fused()
@@ -54,11 +56,11 @@ def test_system_forces_match_oracle(gpu, precision):
     torch.cuda.synchronize()
     got = ar.system_arrays(0)
     q32 = spec.q.astype(np.float32).astype(np.float64)
-    F, T, pairs, r = O.system_forces(spec.x, q32, O.shape_table()["tet10"], spec.L, (11, 6, 4), net.layers_f64())
+    F, T, pairs, r = O.system_forces(spec.x, q32, O.shape_table()[shape_name], spec.L, hyp, net.layers_f64())
     eF = np.max(np.abs(got["F"] - F)) / np.max(np.abs(F))
     eT = np.max(np.abs(got["T"] - T)) / np.max(np.abs(T))
     eU = abs(got["upot"].sum() - r["E"].sum()) / np.sum(np.abs(r["E"]))
-    print(f"system forces ({precision}): {len(pairs)} pairs, F {eF:.2e} T {eT:.2e} U {eU:.2e}")
+    print(f"system forces {shape_name} ({precision}): {len(pairs)} pairs, F {eF:.2e} T {eT:.2e} U {eU:.2e}")
     assert eF < 1e-4 and eT < 1e-4 and eU < 1e-5
     # momentum: Newton's third law holds pair by pair
     assert np.abs(got["F"].sum(0)).max() < 1e-9 * max(1.0, np.abs(got["F"]).sum())
@@ -98,3 +100,73 @@ def test_step_deterministic(gpu):
         outs.append(ar.system_arrays(0))
     for k in "xvqpFT":
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_tuned_launches_give_same_trajectory(gpu, tmp_path):
+    """Skin, gather lanes and fused grid change only FP64 summation order: 30 steps agree to 1e-12."""
+    import nepx
+    from nepx import tune
+    from nepx.configs import make_system
+    from nepx.engine import StepRunner
+    net = nepx.EnergyNet.random(40, [64, 64], seed=3, device=gpu)
+    shape = nepx.get_shape("tet10")
+    old = tune.get_launch()
+    out = []
+    try:
+        for t in (tune.Tuning(), tune.Tuning(skin=0.3, gather_lanes=4, fused_blocks_per_cu=2)):
+            tune.set_launch(t)
+            spec = make_system("tet10", 2000, 0.45, seed=8, skin=t.skin)
+            ar = nepx.Arena([spec], shape, (11, 6, 4), net, device=gpu)
+            r = StepRunner(ar, use_graph=True)
+            r.init()
+            r.run(30)
+            torch.cuda.synchronize()
+            ar.check()
+            out.append(ar.system_arrays(0))
+        for k in ("x", "q", "v", "p"):
+            scale = np.abs(out[0][k]).max()
+            assert np.abs(out[0][k] - out[1][k]).max() <= 1e-12 * scale, k
+        res = tune.Autotuner(str(tmp_path / "tune.json")).tune(
+            [make_system("tet10", 1000, 0.45, seed=9)], shape, (11, 6, 4), net, gpu, steps=10, warmup=3,
+            space={"skin": (0.4, 0.5), "gather_lanes": (8, 16), "fused_blocks_per_cu": (1, 2)})
+        assert res.best.gather_lanes in (8, 16) and len(res.trials) >= 4
+        again = tune.Autotuner(str(tmp_path / "tune.json")).tune(
+            [make_system("tet10", 1000, 0.45, seed=9)], shape, (11, 6, 4), net, gpu)
+        assert again.cached and again.best == res.best
+    finally:
+        tune.set_launch(tune.Tuning(**old))
+
+
+def test_list_overflow_skips_steps_and_recovers(gpu):
+    """A list that overflows makes the integrator skip its steps (state frozen); after the host grows
+    the list the run continues and ends bitwise where an unconstrained run ends."""
+    import nepx
+    from nepx.configs import make_system
+    from nepx.engine import StepRunner
+    net = nepx.EnergyNet.random(40, [64, 64], seed=3, device=gpu)
+    shape = nepx.get_shape("tet10")
+    ref = nepx.Arena([make_system("tet10", 1500, 0.45, seed=5)], shape, (11, 6, 4), net, device=gpu)
+    rr = StepRunner(ref, use_graph=True)
+    rr.init()
+    rr.run(20)
+    torch.cuda.synchronize()
+    ref.check()
+    ar = nepx.Arena([make_system("tet10", 1500, 0.45, seed=5)], shape, (11, 6, 4), net, device=gpu)
+    r = StepRunner(ar, use_graph=True)
+    r.init()
+    ar._alloc_list(1000)          # far too small: the forced build overflows
+    ar.build_list(True)
+    r._capture()
+    r.run(5)
+    torch.cuda.synchronize()
+    assert ar.flags()[1] == 1 and ar.device_steps() == 0
+    x0 = ar.system_arrays(0)["x"].copy()
+    assert r.recover() and ar.cap > 1000
+    assert np.array_equal(ar.system_arrays(0)["x"], x0)        # the skipped steps left the state alone
+    r.run(20 - ar.device_steps())
+    torch.cuda.synchronize()
+    ar.check()
+    assert ar.device_steps() == 20
+    a, b = ref.system_arrays(0), ar.system_arrays(0)
+    for k in ("x", "v", "q", "p", "F", "T"):
+        assert np.array_equal(a[k], b[k]), k

@@ -25,8 +25,9 @@ def test_struct_layouts_match_binding():
     from nepx import _lib
     sizes = (C.c_int32 * 8)()
     assert _lib.lib.nepx_struct_sizes(sizes) == 0
-    assert list(sizes[:6]) == [C.sizeof(_lib.PairModel), C.sizeof(_lib.Net), _lib.SYSPARAM_DTYPE.itemsize,
-                               _lib.THERMO_DTYPE.itemsize, C.sizeof(_lib.State), C.sizeof(_lib.NList)]
+    assert list(sizes[:7]) == [C.sizeof(_lib.PairModel), C.sizeof(_lib.Net), _lib.SYSPARAM_DTYPE.itemsize,
+                               _lib.THERMO_DTYPE.itemsize, C.sizeof(_lib.State), C.sizeof(_lib.NList),
+                               C.sizeof(_lib.Tuning)]
 
 
 def _emulate_mfma_net(packed, d_in, width, n_hidden, G, act):
@@ -258,3 +259,40 @@ def test_interval_allocator_first_fit_and_merge():
     a.release(3, 4)
     a.release(7, 3)
     assert a.free == [(0, 10)]
+
+
+def test_tuning_roundtrip_and_validation():
+    import ctypes
+    from nepx import _lib, tune
+    old = tune.get_launch()
+    try:
+        tune.set_launch(tune.Tuning(gather_lanes=8, fused_blocks_per_cu=2))
+        got = tune.get_launch()
+        assert got["gather_lanes"] == 8 and got["fused_blocks_per_cu"] == 2
+        bad = _lib.Tuning()
+        bad.gather_lanes = 12
+        assert _lib.lib.nepx_set_tuning(ctypes.byref(bad)) == -1
+        assert b"gather_lanes" in _lib.lib.nepx_last_error()
+        tune.set_launch(tune.Tuning(gather_lanes=0, fused_blocks_per_cu=0, pair_blocks_per_cu=0,
+                                    net_blocks_per_cu=0))
+        assert tune.get_launch() == {"fused_blocks_per_cu": 1, "pair_blocks_per_cu": 16, "gather_lanes": 16,
+                                     "net_blocks_per_cu": 2}
+    finally:
+        tune.set_launch(tune.Tuning(**old))
+
+
+def test_coordinate_descent_finds_separable_minimum(tmp_path):
+    from nepx import tune
+    target = {"skin": 0.4, "gather_lanes": 8, "fused_blocks_per_cu": 2}
+
+    def cost(t):
+        return (10 * abs(t.skin - target["skin"]) + abs(t.gather_lanes - target["gather_lanes"]) / 4
+                + abs(t.fused_blocks_per_cu - target["fused_blocks_per_cu"]) + 1.0)
+
+    best, trials = tune.coordinate_descent(cost, tune.DEFAULT_SPACE, tune.Tuning(),
+                                           ["skin", "gather_lanes", "fused_blocks_per_cu"])
+    assert (best.skin, best.gather_lanes, best.fused_blocks_per_cu) == (0.4, 8, 2)
+    assert len(trials) == len({str(t.tuning) for t in trials})        # no candidate measured twice
+    at = tune.Autotuner(str(tmp_path / "t.json"))
+    at._store("k", best)
+    assert at._load()["k"]["gather_lanes"] == 8
