@@ -166,8 +166,9 @@ __device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3
     SiteGeo s;
     mat_vec(R, m.sites[3 * k], m.sites[3 * k + 1], m.sites[3 * k + 2], s.ax, s.ay, s.az);
     const float rx = s.ax + ox, ry = s.ay + oy, rz = s.az + oz;
-    s.r = sqrtf(rx * rx + ry * ry + rz * rz);
-    s.ir = 1.f / s.r;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    s.ir = __builtin_amdgcn_rsqf(r2);  // hardware 1/sqrt (about 1 ulp)
+    s.r = r2 * s.ir;
     s.ux = rx * s.ir;
     s.uy = ry * s.ir;
     s.uz = rz * s.ir;

@@ -77,18 +77,14 @@ struct NetCfg {
     static constexpr NetLayoutH layout_h() { return NetLayoutH{DPH, DP, HP, NL}; }
 };
 
-// tanh from its definition, branch-free: for |z| >= 1/4, (1 - t)/(1 + t) with
-// t = e^{-2|z|} (hardware exp2 and reciprocal; relative error a few 1e-7),
-// below 1/4 the Taylor series z - z^3/3 + 2z^5/15 - 17z^7/315 + 62z^9/2835
-// (truncation error below 1e-8 relative).
+// tanh from its definition, branch-free: sign(z) (1 - t)/(1 + t) with
+// t = e^{-2|z|} (hardware exp2 and reciprocal).  The absolute error stays
+// below 2e-7 for all z (relative error grows as |z| -> 0, where 1 - t
+// cancels); the network consumes activations only through products with
+// O(1) weights, so the absolute error is the one that matters.
 __device__ __forceinline__ float tanh_act(float z) {
-    const float a = fabsf(z);
-    const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * a);  // -2 log2(e) |z|
-    const float r = (1.f - t) * __builtin_amdgcn_rcpf(1.f + t);
-    const float z2 = z * z;
-    const float p =
-        z * (1.f + z2 * (-1.f / 3.f + z2 * (2.f / 15.f + z2 * (-17.f / 315.f + z2 * (62.f / 2835.f)))));
-    return a < 0.25f ? p : copysignf(r, z);
+    const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * fabsf(z));  // -2 log2(e) |z|
+    return copysignf((1.f - t) * __builtin_amdgcn_rcpf(1.f + t), z);
 }
 
 __device__ __forceinline__ float act_fwd(float z, int act) {

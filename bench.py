@@ -23,20 +23,39 @@ PEAK_FP16_MFMA_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
-def profiled_traffic(workload):
-    """HBM bytes per launch of the fused pair kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy), or None when no summary exists."""
+def profiled(workload):
+    """Per-launch figures of the fused pair kernel from the committed rocprofv3 summaries:
+    HBM traffic (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy), VALU issue utilisation and
+    MFMA busy fraction (SQ counters).  Missing summaries give None."""
     import re
-    path = os.path.join(ROOT, "profiles", f"k_fused_{workload}_hbm_counters.md")
-    if not os.path.exists(path):
-        return None, None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
-    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+    out = {"traffic": None, "valu_issue_util": None, "mfma_busy": None, "source": []}
+    for name, keys in ((f"k_fused_{workload}_hbm_counters.md", ("traffic_bytes_per_launch",)),
+                       (f"k_fused_{workload}_sq_counters.md", ("valu_issue_util", "mfma_busy"))):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        text = open(path).read()
+        for k in keys:
+            m = re.search(rf"^{k}:\s*([0-9.eE+-]+)", text, re.M)
+            if m:
+                out["traffic" if k.startswith("traffic") else k] = float(m.group(1))
+        out["source"].append(os.path.relpath(path, ROOT))
+    return out
 
 
 def ffnn_flops_per_pair(D, widths):
     h1, h2 = widths
     return 4 * (D * h1 + h1 * h2 + h2)
+
+
+def pair_flops(n_sites, hyp, D, widths):
+    """Algorithmic FP32 FLOPs per pair of the fused kernel, counting its contractions: the angular
+    accumulation A[n][lm] += f_n Z_lm (2 (nang+1) NLM per site), the adjoint's c_lm and d_lm
+    (4 (nang+1) NLM per site) and the network forward + backward-to-input 4 (D H1 + H1 H2 + H2).
+    Site geometry, radial basis and harmonics recurrences are not counted."""
+    nrad, nang, lmax = hyp
+    nlm = lmax * (lmax + 2)
+    return 2 * n_sites * 6 * (nang + 1) * nlm + ffnn_flops_per_pair(D, widths)
 
 
 def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
@@ -273,12 +292,12 @@ def main():
     P = int(flags[3])
     th = ar.thermo()[0]
 
-    flops = ffnn_flops_per_pair(D, widths) * P
+    flops = pair_flops(shape.n_sites, hyp, D, widths) * P
     achieved = flops / (fused_ms * 1e-3) / 1e12
     half = net.precision == "f16x3"
-    peak = PEAK_FP16_MFMA_TFLOPS if half else PEAK_FP32_MFMA_TFLOPS
+    peak = PEAK_FP32_MFMA_TFLOPS
     mfma_desc = "fp16 hi/lo split MFMA (3 products per FP32 product, FP32 accumulate)" if half else "MFMA f32"
-    traffic, traffic_src = profiled_traffic(a.workload)
+    prof = profiled(a.workload)
     res = {
         "metric": ("particle-steps/s at 32k tetrahedra" if a.workload == "tet32k"
                    else f"particle-steps/s ({a.workload})"),
@@ -298,9 +317,14 @@ def main():
                    "particles": spec.n, "pairs_per_step": P, "hipgraph": bool(a.graph),
                    "list_builds": int(flags[4]), "wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
+                     "frac": achieved / peak, "traffic": prof["traffic"],
                      "kernel": "k_fused (descriptor + FFNN fwd/bwd + adjoint)", "kernel_ms": fused_ms,
-                     "flops_per_launch": flops},
+                     "compute": "FP32 VALU (descriptor, adjoint) + MFMA (network); the bound is FP32 compute",
+                     "flops_per_launch": flops, "flops_per_pair": pair_flops(shape.n_sites, hyp, D, widths),
+                     "ffnn_flops_per_launch": ffnn_flops_per_pair(D, widths) * P,
+                     "algorithmic_bytes_per_launch": P * 72 + spec.n * 16,
+                     "valu_issue_util": prof["valu_issue_util"], "mfma_busy": prof["mfma_busy"],
+                     "profile_source": prof["source"]},
         "thermo": {"Kt_per_dof": float(th["Kt"] / spec.ndof * 2), "U": float(th["U"])},
         "tuning": tuning,
     }
