@@ -113,17 +113,30 @@ __global__ void k_cells_fill(nepx_state st, nepx_nlist nl) {
     nl.cell_list[pos] = i;
 }
 
-// Visit every candidate j != i of particle i (27 cells, or the whole system
-// when the box is too small for three cells a side), calling f(j, d2, r2).
+// The list build gives each particle a group of kLG lanes.  The lanes walk
+// the candidates j of particle i (27 cells, or the whole system when the
+// box is too small for three cells a side) in rounds of kLG: round r of a
+// cell covers its entries r*kLG + sub.  Every lane of a group sees the same
+// cells and rounds (same i), so ballots inside a round are uniform over the
+// group; f(j, valid) is called for each round slot, with valid = false for
+// slots past the end of a cell and for j == i.
+constexpr int kLG = 16;
+
+__device__ __forceinline__ uint32_t group_mask(uint64_t ballot) {
+    const int gbase = (threadIdx.x & 63) & ~(kLG - 1);
+    return (uint32_t)((ballot >> gbase) & ((1ull << kLG) - 1ull));
+}
+
 template <class F>
-__device__ __forceinline__ void for_candidates(const nepx_state& st, const nepx_nlist& nl, int i, F&& f) {
+__device__ __forceinline__ void for_candidates(const nepx_state& st, const nepx_nlist& nl, int i, int sub, F&& f) {
     const int s = st.sys[i];
     const nepx_sysparam& sp = st.sp[s];
     const double L = sp.L;
     if (sp.nc <= 0) {
-        for (int j = sp.p_start; j < sp.p_end; ++j) {
-            if (j == i || st.sys[j] != s) continue;
-            f(j);
+        for (int base = sp.p_start; base < sp.p_end; base += kLG) {
+            const int j = base + sub;
+            const bool valid = j < sp.p_end && j != i && st.sys[j] == s;
+            f(valid ? j : i, valid);
         }
         return;
     }
@@ -137,34 +150,47 @@ __device__ __forceinline__ void for_candidates(const nepx_state& st, const nepx_
                 const int az = (cz + dz + nc) % nc;
                 const int c = sp.cell_off + (ax * nc + ay) * nc + az;
                 const int b = nl.cell_start[c], e = nl.cell_start[c + 1];
-                for (int k = b; k < e; ++k) {
-                    const int j = nl.cell_list[k];
-                    if (j != i) f(j);
+                for (int base = b; base < e; base += kLG) {
+                    const int k = base + sub;
+                    const int j = k < e ? nl.cell_list[k] : i;
+                    f(j, j != i);
                 }
             }
         }
     }
 }
 
-__global__ void k_nbr_count(nepx_state st, nepx_nlist nl) {
+template <int W>
+__device__ __forceinline__ int group_sum(int v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+__global__ __launch_bounds__(kNB) void k_nbr_count(nepx_state st, nepx_nlist nl) {
     if (!want_build(nl.flags)) return;
-    const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i >= st.n) return;
+    const int i = (blockIdx.x * kNB + threadIdx.x) / kLG;
+    const int sub = threadIdx.x % kLG;
+    if (i >= st.n) return;  // whole groups exit together
     int nf = 0, nr = 0;
     if (live(st, i)) {
         const nepx_sysparam& sp = st.sp[st.sys[i]];
         const double r2 = __dmul_rn(sp.r_list, sp.r_list);
-        for_candidates(st, nl, i, [&](int j) {
+        for_candidates(st, nl, i, sub, [&](int j, bool valid) {
             double tx, ty, tz;
-            if (pair_d2(st, sp.L, i, j, tx, ty, tz) < r2) {
+            if (valid && pair_d2(st, sp.L, i, j, tx, ty, tz) < r2) {
                 if (j > i) ++nf;
                 else ++nr;
             }
         });
     }
-    nl.row_count[i] = nf;
-    nl.rrow_count[i] = nr;
-    if (nf > kSortMax || nr > kSortMax) atomicMax(&nl.flags[F_MAXROW], nf > nr ? nf : nr);
+    nf = group_sum<kLG>(nf);
+    nr = group_sum<kLG>(nr);
+    if (sub == 0) {
+        nl.row_count[i] = nf;
+        nl.rrow_count[i] = nr;
+        if (nf > kSortMax || nr > kSortMax) atomicMax(&nl.flags[F_MAXROW], nf > nr ? nf : nr);
+    }
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_rows(nepx_nlist nl, int n) {
@@ -181,30 +207,39 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_rows(nepx_nlist nl, int n) 
     }
 }
 
-__global__ void k_nbr_fill(nepx_state st, nepx_nlist nl) {
+// Same walk as k_nbr_count; each round's accepted candidates are placed by a
+// ballot prefix within the group (no atomics).  The row order this gives
+// depends on the cell lists; k_sort_rows makes it canonical.
+__global__ __launch_bounds__(kNB) void k_nbr_fill(nepx_state st, nepx_nlist nl) {
     if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
-    const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i >= st.n || !live(st, i)) return;
+    const int i = (blockIdx.x * kNB + threadIdx.x) / kLG;
+    const int sub = threadIdx.x % kLG;
+    if (i >= st.n || !live(st, i)) return;  // uniform per group
     const nepx_sysparam& sp = st.sp[st.sys[i]];
     const double r2 = __dmul_rn(sp.r_list, sp.r_list);
+    const uint32_t below = (1u << sub) - 1u;
     int pf = nl.row_start[i], pr = nl.rrow_start[i];
-    for_candidates(st, nl, i, [&](int j) {
+    for_candidates(st, nl, i, sub, [&](int j, bool valid) {
         double tx, ty, tz;
-        if (pair_d2(st, sp.L, i, j, tx, ty, tz) < r2) {
-            if (j > i) {
-                nl.nbr[pf] = j;
-                nl.slot_row[pf] = i;
-                ++pf;
-            } else {
-                nl.rnbr[pr++] = j;
-            }
+        const bool in = valid && pair_d2(st, sp.L, i, j, tx, ty, tz) < r2;
+        const bool fwd = in && j > i, rev = in && j < i;
+        const uint32_t mf = group_mask(__ballot(fwd)), mr = group_mask(__ballot(rev));
+        if (fwd) {
+            const int slot = pf + __popc(mf & below);
+            nl.nbr[slot] = j;
+            nl.slot_row[slot] = i;
         }
+        if (rev) nl.rnbr[pr + __popc(mr & below)] = j;
+        pf += __popc(mf);
+        pr += __popc(mr);
     });
 }
 
-// Sort each row ascending: blocks of 256 threads stride over the rows, bitonic in LDS.
-// The grid is capped, so the launch costs little when no rebuild is due.
-__global__ __launch_bounds__(256) void k_sort_rows(nepx_nlist nl, int n, int reverse) {
+// Sort each row ascending: one wave per row (blocks of 64 stride over the
+// rows), bitonic in LDS.  The grid is capped, so the launch costs little when
+// no rebuild is due.
+constexpr int kSortThreads = 64;
+__global__ __launch_bounds__(kSortThreads) void k_sort_rows(nepx_nlist nl, int n, int reverse) {
     if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
     __shared__ int32_t buf[kSortMax];
     const int32_t* start = reverse ? nl.rrow_start : nl.row_start;
@@ -214,11 +249,11 @@ __global__ __launch_bounds__(256) void k_sort_rows(nepx_nlist nl, int n, int rev
         if (len <= 1) continue;  // uniform per block
         int p2 = 1;
         while (p2 < len) p2 <<= 1;
-        for (int k = threadIdx.x; k < p2; k += 256) buf[k] = k < len ? data[b + k] : 0x7fffffff;
+        for (int k = threadIdx.x; k < p2; k += kSortThreads) buf[k] = k < len ? data[b + k] : 0x7fffffff;
         __syncthreads();
         for (int size = 2; size <= p2; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int k = threadIdx.x; k < p2; k += 256) {
+                for (int k = threadIdx.x; k < p2; k += kSortThreads) {
                     const int partner = k ^ stride;
                     if (partner > k) {
                         const bool up = (k & size) == 0;
@@ -232,17 +267,19 @@ __global__ __launch_bounds__(256) void k_sort_rows(nepx_nlist nl, int n, int rev
                 __syncthreads();
             }
         }
-        for (int k = threadIdx.x; k < len; k += 256) data[b + k] = buf[k];
+        for (int k = threadIdx.x; k < len; k += kSortThreads) data[b + k] = buf[k];
         __syncthreads();
     }
 }
 
-// Reverse entries hold the partner i; replace it by the slot of (i, k) in row i.
-__global__ void k_rev_slots(nepx_nlist nl, int n) {
+// Reverse entries hold the partner i; replace it by the slot of (i, k) in row i
+// (binary search in the sorted row).  kLG lanes per particle k.
+__global__ __launch_bounds__(kNB) void k_rev_slots(nepx_nlist nl, int n) {
     if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
-    const int k = blockIdx.x * kNB + threadIdx.x;
+    const int k = (blockIdx.x * kNB + threadIdx.x) / kLG;
+    const int sub = threadIdx.x % kLG;
     if (k >= n) return;
-    for (int e = nl.rrow_start[k]; e < nl.rrow_start[k + 1]; ++e) {
+    for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += kLG) {
         const int i = nl.rnbr[e];
         int lo = nl.row_start[i], hi = nl.row_start[i + 1] - 1, slot = -1;
         while (lo <= hi) {
@@ -504,13 +541,14 @@ extern "C" int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, v
     hipLaunchKernelGGL(k_cells_count, dim3(gb), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(kScanBlock), 0, s, *nl);
     hipLaunchKernelGGL(k_cells_fill, dim3(gb), dim3(kNB), 0, s, *st, *nl);
-    hipLaunchKernelGGL(k_nbr_count, dim3(gb), dim3(kNB), 0, s, *st, *nl);
+    const int gg = blocks_for((int64_t)n * kLG, kNB);
+    hipLaunchKernelGGL(k_nbr_count, dim3(gg), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_scan_rows, dim3(2), dim3(kScanBlock), 0, s, *nl, n);
-    hipLaunchKernelGGL(k_nbr_fill, dim3(gb), dim3(kNB), 0, s, *st, *nl);
-    const int gs = n < 4096 ? n : 4096;
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 0);
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 1);
-    hipLaunchKernelGGL(k_rev_slots, dim3(gb), dim3(kNB), 0, s, *nl, n);
+    hipLaunchKernelGGL(k_nbr_fill, dim3(gg), dim3(kNB), 0, s, *st, *nl);
+    const int gs = n < 16384 ? n : 16384;
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 0);
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 1);
+    hipLaunchKernelGGL(k_rev_slots, dim3(gg), dim3(kNB), 0, s, *nl, n);
     hipLaunchKernelGGL(k_build_done, dim3(gb), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
     NEPX_LAUNCH_CHECK();
@@ -547,10 +585,10 @@ extern "C" int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64
     if (m > 0)
         hipLaunchKernelGGL(k_pairs_fill, dim3(blocks_for(m, kNB)), dim3(kNB), 0, s, *nl, pairs, m,
                            nl->cell_fill, nl->cell_fill + n);
-    const int gs = n < 4096 ? n : 4096;
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 0);
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(256), 0, s, *nl, n, 1);
-    hipLaunchKernelGGL(k_rev_slots, dim3(blocks_for(n, kNB)), dim3(kNB), 0, s, *nl, n);
+    const int gs = n < 16384 ? n : 16384;
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 0);
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 1);
+    hipLaunchKernelGGL(k_rev_slots, dim3(blocks_for((int64_t)n * kLG, kNB)), dim3(kNB), 0, s, *nl, n);
     hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
