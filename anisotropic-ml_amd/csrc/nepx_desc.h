@@ -18,6 +18,8 @@
 // dQ_l^m/dz = (l+m) Q_{l-1}^m  (A. Bigi et al., "sphericart", J. Chem. Phys.
 // 159, 064802 (2023)).
 #pragma once
+#include <type_traits>
+
 #include "nepx_common.h"
 
 namespace nepx {
@@ -81,11 +83,14 @@ struct K2Tab {
 };
 
 // Floats of angular accumulators one lane holds in a pass over the sites.
+// The l range 1..LMAX is cut greedily into passes of at most this many
+// accumulators ((nang+1) x lm count); an l that alone exceeds it gets a pass
+// of its own.
 constexpr int kAccBudget = 176;
-constexpr int split_l(int nang, int lmax) {
-    int L = lmax;
-    while (L > 1 && (nang + 1) * L * (L + 2) > kAccBudget) --L;
-    return L;
+constexpr int pass_end(int nang, int lmax, int l0) {
+    int l1 = l0;
+    while (l1 < lmax && (nang + 1) * ((l1 + 2) * (l1 + 2) - l0 * l0) <= kAccBudget) ++l1;
+    return l1;
 }
 
 // Slot of (l, m, component) in the NLM accumulators: l*l-1 + {0: m = 0, 2m-1: cos, 2m: sin}.
@@ -100,12 +105,8 @@ struct DescCfg {
     static constexpr int NLM = LMAX * (LMAX + 2);
     static constexpr int D = (NRAD + 1) + (NANG + 1) * LMAX;
     static_assert(NANG <= NRAD, "nang must not exceed nrad");
-    // The angular accumulators of l = 1..L_SPLIT are held per lane in one
-    // pass; any higher l go to a second pass over the sites.
-    static constexpr int L_SPLIT = split_l(NANG, LMAX);
-    static constexpr bool ONE_PASS = L_SPLIT == LMAX;
-    static_assert(ONE_PASS || (NANG + 1) * (LMAX * (LMAX + 2) - L_SPLIT * (L_SPLIT + 2)) <= 2 * kAccBudget,
-                  "second l-range too wide");
+    // One pass over the sites holds every angular accumulator.
+    static constexpr bool ONE_PASS = pass_end(NANG, LMAX, 1) >= LMAX;
 };
 
 // A contiguous range of l: accumulator slots lm_slot(l, m, part) - OFF.
@@ -115,10 +116,14 @@ struct LRange {
     static constexpr int OFF = L0 * L0 - 1;
     static constexpr int N = (L1 + 1) * (L1 + 1) - L0 * L0;
 };
-template <class Cfg>
-using Pass0 = LRange<1, Cfg::L_SPLIT>;
-template <class Cfg>
-using Pass1 = LRange<Cfg::L_SPLIT + 1, Cfg::LMAX>;
+
+// Calls fn(LRange<...>{}, first) for each pass of Cfg's l range, in order.
+template <class Cfg, int L0 = 1, class Fn>
+__device__ __forceinline__ void for_passes(Fn&& fn) {
+    constexpr int L1 = pass_end(Cfg::NANG, Cfg::LMAX, L0);
+    fn(LRange<L0, L1>{}, std::integral_constant<bool, L0 == 1>{});
+    if constexpr (L1 < Cfg::LMAX) for_passes<Cfg, L1 + 1>(fn);
+}
 
 // One pair in the frame centred on particle i: orientations, t = X_j - X_i,
 // and the centroid of the 2*Ns sites,
@@ -176,12 +181,72 @@ __device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3
     return s;
 }
 
-// Forward accumulators over the l of range LR: A[n][lm - LR::OFF] = sum_a f_n(a) Z_lm(u_a),
-// and (RAD) g_rad[n] = sum_a f_n(a).
-template <class Cfg, class LR, bool RAD>
-__device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const PairIn& p,
-                                                float (&grad)[Cfg::NRAD + 1],
-                                                float (&A)[Cfg::NANG + 1][LR::N]) {
+// Site visitors.  visit(site, body_done) calls site(s, w) for every site
+// (geometry relative to the cloud centroid, per-site weight w) and
+// body_done(b) after the sites of body b.
+template <class Cfg>
+struct PairSites {  // the 2 Ns sites of one pair, bodies i (0) and j (1), weight 1
+    const nepx_pair_model& m;
+    const PairIn& p;
+    float inv_rn;
+    template <class S, class B>
+    __device__ __forceinline__ void visit(S&& site, B&& body_done) const {
+#pragma unroll 1
+        for (int body = 0; body < 2; ++body) {
+            const float4 q = body ? p.qj : p.qi;
+            const Mat3 R = quat_matrix(q.x, q.y, q.z, q.w);
+            const float ox = (body ? p.tx : 0.f) - p.cx, oy = (body ? p.ty : 0.f) - p.cy,
+                        oz = (body ? p.tz : 0.f) - p.cz;
+#pragma unroll 1
+            for (int k = 0; k < Cfg::NS; ++k) site(site_geo(m, R, k, ox, oy, oz, inv_rn), 1.f);
+            body_done(body);
+        }
+    }
+};
+
+struct CloudSites {  // nd points of a site cloud (xyz interleaved), optional per-point weights
+    const float* __restrict__ pts;
+    const float* __restrict__ w;
+    int nd;
+    float cx, cy, cz, inv_rn;
+    __device__ __forceinline__ CloudSites(const float* pts_, const float* w_, int nd_, float inv_rn_)
+        : pts(pts_), w(w_), nd(nd_), inv_rn(inv_rn_) {
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int a = 0; a < nd; ++a) {
+            sx += pts[3 * a];
+            sy += pts[3 * a + 1];
+            sz += pts[3 * a + 2];
+        }
+        const float inv = 1.f / (float)nd;
+        cx = sx * inv;
+        cy = sy * inv;
+        cz = sz * inv;
+    }
+    template <class S, class B>
+    __device__ __forceinline__ void visit(S&& site, B&& body_done) const {
+#pragma unroll 1
+        for (int a = 0; a < nd; ++a) {
+            SiteGeo s;
+            s.ax = s.ay = s.az = 0.f;
+            const float rx = pts[3 * a] - cx, ry = pts[3 * a + 1] - cy, rz = pts[3 * a + 2] - cz;
+            const float r2 = rx * rx + ry * ry + rz * rz;
+            s.ir = __builtin_amdgcn_rsqf(r2);
+            s.r = r2 * s.ir;
+            s.ux = rx * s.ir;
+            s.uy = ry * s.ir;
+            s.uz = rz * s.ir;
+            s.rho = s.r * inv_rn;
+            site(s, w ? w[a] : 1.f);
+        }
+        body_done(0);
+    }
+};
+
+// Forward accumulators over the l of range LR: A[n][lm - LR::OFF] = sum_a w_a f_n(a) Z_lm(u_a),
+// and (RAD) g_rad[n] = sum_a w_a f_n(a).
+template <class Cfg, class LR, bool RAD, class Sites>
+__device__ __forceinline__ void accumulate_sites(const Sites& sites, float (&grad)[Cfg::NRAD + 1],
+                                                 float (&A)[Cfg::NANG + 1][LR::N]) {
     if constexpr (RAD) {
 #pragma unroll
         for (int n = 0; n <= Cfg::NRAD; ++n) grad[n] = 0.f;
@@ -190,18 +255,10 @@ __device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const 
     for (int n = 0; n <= Cfg::NANG; ++n)
 #pragma unroll
         for (int k = 0; k < LR::N; ++k) A[n][k] = 0.f;
-    const float inv_rn = 1.f / m.r_nep;
-#pragma unroll 1
-    for (int body = 0; body < 2; ++body) {
-        const float4 q = body ? p.qj : p.qi;
-        const Mat3 R = quat_matrix(q.x, q.y, q.z, q.w);
-        const float ox = (body ? p.tx : 0.f) - p.cx, oy = (body ? p.ty : 0.f) - p.cy,
-                    oz = (body ? p.tz : 0.f) - p.cz;
-#pragma unroll 1
-        for (int k = 0; k < Cfg::NS; ++k) {
-            const SiteGeo s = site_geo(m, R, k, ox, oy, oz, inv_rn);
+    sites.visit(
+        [&](const SiteGeo& s, float w) {
             const float x = 2.f * (s.rho - 1.f) * (s.rho - 1.f) - 1.f;
-            const float hf = 0.25f * (1.f + cospif(s.rho));  // fc / 2
+            const float hf = 0.25f * (1.f + cospif(s.rho)) * w;  // w fc / 2
             float fa[Cfg::NANG + 1];
             float t0 = 1.f, t1 = x;
 #pragma unroll
@@ -234,8 +291,15 @@ __device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const 
                     }
                 }
             }
-        }
-    }
+        },
+        [](int) {});
+}
+
+template <class Cfg, class LR, bool RAD>
+__device__ __forceinline__ void desc_accumulate(const nepx_pair_model& m, const PairIn& p,
+                                                float (&grad)[Cfg::NRAD + 1],
+                                                float (&A)[Cfg::NANG + 1][LR::N]) {
+    accumulate_sites<Cfg, LR, RAD>(PairSites<Cfg>{m, p, 1.f / m.r_nep}, grad, A);
 }
 
 // G entries of range LR: g_ang[n][l] = sum_m K_lm^2 A[n][lm]^2, at G[NRAD+1 + n*LMAX + l-1].
@@ -305,17 +369,10 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
         }
     }
     const float inv_rn = 1.f / m.r_nep;
-#pragma unroll 1
-    for (int body = 0; body < 2; ++body) {
-        const float4 q = body ? p.qj : p.qi;
-        const Mat3 R = quat_matrix(q.x, q.y, q.z, q.w);
-        const float ox = (body ? p.tx : 0.f) - p.cx, oy = (body ? p.ty : 0.f) - p.cy,
-                    oz = (body ? p.tz : 0.f) - p.cz;
-        float gx_s = 0.f, gy_s = 0.f, gz_s = 0.f, tx_s = 0.f, ty_s = 0.f, tz_s = 0.f;
-        float ax_s = 0.f, ay_s = 0.f, az_s = 0.f;
-#pragma unroll 1
-        for (int k = 0; k < Cfg::NS; ++k) {
-            const SiteGeo s = site_geo(m, R, k, ox, oy, oz, inv_rn);
+    float gx_s = 0.f, gy_s = 0.f, gz_s = 0.f, tx_s = 0.f, ty_s = 0.f, tz_s = 0.f;
+    float ax_s = 0.f, ay_s = 0.f, az_s = 0.f;
+    PairSites<Cfg>{m, p, inv_rn}.visit(
+        [&](const SiteGeo& s, float) {
             const float xm = s.rho - 1.f;
             const float x = 2.f * xm * xm - 1.f;
             float sp, cp;
@@ -410,17 +467,19 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
                 ay_s += s.ay;
                 az_s += s.az;
             }
-        }
-        if (body == 0) {
-            acc.G0x += gx_s; acc.G0y += gy_s; acc.G0z += gz_s;
-            acc.T0x += tx_s; acc.T0y += ty_s; acc.T0z += tz_s;
-            acc.A0x += ax_s; acc.A0y += ay_s; acc.A0z += az_s;
-        } else {
-            acc.G1x += gx_s; acc.G1y += gy_s; acc.G1z += gz_s;
-            acc.T1x += tx_s; acc.T1y += ty_s; acc.T1z += tz_s;
-            acc.A1x += ax_s; acc.A1y += ay_s; acc.A1z += az_s;
-        }
-    }
+        },
+        [&](int body) {
+            if (body == 0) {
+                acc.G0x += gx_s; acc.G0y += gy_s; acc.G0z += gz_s;
+                acc.T0x += tx_s; acc.T0y += ty_s; acc.T0z += tz_s;
+                acc.A0x += ax_s; acc.A0y += ay_s; acc.A0z += az_s;
+            } else {
+                acc.G1x += gx_s; acc.G1y += gy_s; acc.G1z += gz_s;
+                acc.T1x += tx_s; acc.T1y += ty_s; acc.T1z += tz_s;
+                acc.A1x += ax_s; acc.A1y += ay_s; acc.A1z += az_s;
+            }
+            gx_s = gy_s = gz_s = tx_s = ty_s = tz_s = ax_s = ay_s = az_s = 0.f;
+        });
 }
 
 // dE/dp_a = g_a - gbar ; F_i = -sum_{a in i} dE/dp_a ; tau = -sum arm x dE/dp

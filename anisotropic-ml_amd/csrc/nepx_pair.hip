@@ -68,32 +68,38 @@ __device__ __forceinline__ void store_out(float* __restrict__ out, int idx, cons
     o4[2] = make_float4(clipf(o.tjx, clip), clipf(o.tjy, clip), clipf(o.tjz, clip), 0.f);
 }
 
-// Descriptor row of one pair, in one or two passes over the sites.
-template <class Cfg, class Row>
-__device__ __forceinline__ void desc_forward(const nepx_pair_model& m, const PairIn& g, Row&& G) {
+// Descriptor row of one pair (or cloud), one pass over the sites per l range.
+template <class Cfg, class Sites, class Row>
+__device__ __forceinline__ void desc_forward_sites(const Sites& sites, Row&& G) {
     float grad[Cfg::NRAD + 1];
-    {
-        float A[Cfg::NANG + 1][Pass0<Cfg>::N];
-        desc_accumulate<Cfg, Pass0<Cfg>, true>(m, g, grad, A);
-        desc_finish<Cfg, Pass0<Cfg>>(A, G);
-    }
+    for_passes<Cfg>([&](auto range, auto first) {
+        using LR = decltype(range);
+        float A[Cfg::NANG + 1][LR::N];
+        accumulate_sites<Cfg, LR, decltype(first)::value>(sites, grad, A);
+        desc_finish<Cfg, LR>(A, G);
+        if constexpr (decltype(first)::value) {
 #pragma unroll
-    for (int n = 0; n <= Cfg::NRAD; ++n) G[n] = grad[n];
-    if constexpr (!Cfg::ONE_PASS) {
-        float A[Cfg::NANG + 1][Pass1<Cfg>::N];
-        desc_accumulate<Cfg, Pass1<Cfg>, false>(m, g, grad, A);
-        desc_finish<Cfg, Pass1<Cfg>>(A, G);
-    }
+            for (int n = 0; n <= Cfg::NRAD; ++n) G[n] = grad[n];
+        }
+    });
 }
 
-// Adjoint pass over range LR, recomputing that range's forward accumulators.
-template <class Cfg, class LR, bool RAD>
-__device__ __forceinline__ void adjoint_pass(const nepx_pair_model& m, const PairIn& g, const float (&dG)[Cfg::D],
-                                             AdjAcc& acc) {
-    float grad[Cfg::NRAD + 1];
-    float A[Cfg::NANG + 1][LR::N];
-    desc_accumulate<Cfg, LR, false>(m, g, grad, A);
-    desc_adjoint<Cfg, LR, RAD>(m, g, dG, A, acc);
+template <class Cfg, class Row>
+__device__ __forceinline__ void desc_forward(const nepx_pair_model& m, const PairIn& g, Row&& G) {
+    desc_forward_sites<Cfg>(PairSites<Cfg>{m, g, 1.f / m.r_nep}, G);
+}
+
+// Adjoint over every l range, recomputing each range's forward accumulators.
+template <class Cfg>
+__device__ __forceinline__ void adjoint_all(const nepx_pair_model& m, const PairIn& g, const float (&dG)[Cfg::D],
+                                            AdjAcc& acc) {
+    for_passes<Cfg>([&](auto range, auto first) {
+        using LR = decltype(range);
+        float grad[Cfg::NRAD + 1];
+        float A[Cfg::NANG + 1][LR::N];
+        desc_accumulate<Cfg, LR, false>(m, g, grad, A);
+        desc_adjoint<Cfg, LR, decltype(first)::value>(m, g, dG, A, acc);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -109,9 +115,13 @@ __global__ __launch_bounds__(kBlock) void k_desc_fwd(nepx_pair_model m, const fl
         float4 qi, qj, t;
         load_pair(qf, pi, pj, pt, idx, qi, qj, t);
         const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
-        float Gr[Cfg::D];
-        desc_forward<Cfg>(m, g, Gr);
-        store_row<Cfg>(G + (size_t)idx * Cfg::D, Gr);
+        if constexpr (Cfg::ONE_PASS) {
+            float Gr[Cfg::D];
+            desc_forward<Cfg>(m, g, Gr);
+            store_row<Cfg>(G + (size_t)idx * Cfg::D, Gr);
+        } else {  // several passes: each writes its entries (no D-float row held in registers)
+            desc_forward<Cfg>(m, g, G + (size_t)idx * Cfg::D);
+        }
     }
 }
 
@@ -132,8 +142,7 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_mode
         float dG[Cfg::D];
         load_row<Cfg>(dEdG + (size_t)idx * Cfg::D, dG);
         AdjAcc acc;
-        adjoint_pass<Cfg, Pass0<Cfg>, true>(m, g, dG, acc);
-        if constexpr (!Cfg::ONE_PASS) adjoint_pass<Cfg, Pass1<Cfg>, false>(m, g, dG, acc);
+        adjoint_all<Cfg>(m, g, dG, acc);
         store_out(out, idx, adj_finish<Cfg>(acc), E ? E[idx] : 0.f, clip);
     }
 }
@@ -282,18 +291,20 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
         // one-pass shapes keep the forward accumulators for the adjoint
         float A[Cfg::ONE_PASS ? Cfg::NANG + 1 : 1][Cfg::ONE_PASS ? Cfg::NLM : 1];
         {
-            float Gr[Cfg::D];
             if constexpr (Cfg::ONE_PASS) {
+                float Gr[Cfg::D];
                 float grad[Cfg::NRAD + 1];
-                desc_accumulate<Cfg, Pass0<Cfg>, true>(m, g, grad, A);
-                desc_finish<Cfg, Pass0<Cfg>>(A, Gr);
+                desc_accumulate<Cfg, LRange<1, Cfg::LMAX>, true>(m, g, grad, A);
+                desc_finish<Cfg, LRange<1, Cfg::LMAX>>(A, Gr);
 #pragma unroll
                 for (int n = 0; n <= Cfg::NRAD; ++n) Gr[n] = grad[n];
-            } else {
-                desc_forward<Cfg>(m, g, Gr);
-            }
 #pragma unroll
-            for (int k = 0; k < Cfg::D; ++k) myrow[k] = valid ? Gr[k] : 0.f;
+                for (int k = 0; k < Cfg::D; ++k) myrow[k] = valid ? Gr[k] : 0.f;
+            } else {  // several passes: each writes its entries straight into the LDS row
+                desc_forward<Cfg>(m, g, myrow);
+                if (!valid)
+                    for (int k = 0; k < Cfg::D; ++k) myrow[k] = 0.f;
+            }
         }
         __syncthreads();
 #pragma unroll 1
@@ -320,12 +331,8 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
 #pragma unroll
             for (int k = 0; k < Cfg::D; ++k) dG[k] = myrow[k];
             AdjAcc acc;
-            if constexpr (Cfg::ONE_PASS) {
-                desc_adjoint<Cfg, Pass0<Cfg>, true>(m, g, dG, A, acc);
-            } else {
-                adjoint_pass<Cfg, Pass0<Cfg>, true>(m, g, dG, acc);
-                adjoint_pass<Cfg, Pass1<Cfg>, false>(m, g, dG, acc);
-            }
+            if constexpr (Cfg::ONE_PASS) desc_adjoint<Cfg, LRange<1, Cfg::LMAX>, true>(m, g, dG, A, acc);
+            else adjoint_all<Cfg>(m, g, dG, acc);
             store_out(out, idx, adj_finish<Cfg>(acc), etile[threadIdx.x], clip);
         }
         __syncthreads();
@@ -399,10 +406,13 @@ using CfgTet10 = DescCfg<10, 11, 6, 4>;    // tetrahedron, 4 vertices + 6 mid-ed
 using CfgCyl12 = DescCfg<12, 17, 6, 6>;    // twisted cylinder, 12 helix sites, D = 60
 using CfgCubeV2 = DescCfg<6, 10, 8, 4>;    // reference cube_v2 with hypers 10 8 4, D = 47
 using CfgThV1 = DescCfg<4, 10, 8, 4>;      // reference th_v1 with hypers 10 8 4, D = 47
+using CfgCubeV2L10 = DescCfg<6, 10, 10, 10>;  // reference cube_v2 with hypers 10 10 10, D = 121
+using CfgThV1L10 = DescCfg<4, 10, 10, 10>;    // reference th_v1 with hypers 10 10 10, D = 121
 
 using Net40x64 = NetCfg<48, 64, 2>;        // 40 -> 64 -> 64 -> 1
 using Net60x128 = NetCfg<64, 128, 2>;      // 60 -> 128 -> 128 -> 1
 using Net47x150 = NetCfg<48, 160, 4>;      // reference TheEnergyNet(150, 3, 47): 47 -> 150 x4 -> 1
+using Net121x150 = NetCfg<128, 160, 4>;    // reference TheEnergyNet(150, 3, 121): 121 -> 150 x4 -> 1
 
 static int grid_for_cap(int64_t units, int per_block, int64_t cap) {
     int64_t g = (units + per_block - 1) / per_block;
@@ -420,7 +430,9 @@ static int grid_for(int64_t maxp, int per_block) {
     X(10, 11, 6, 4, CfgTet10)                       \
     X(12, 17, 6, 6, CfgCyl12)                       \
     X(6, 10, 8, 4, CfgCubeV2)                       \
-    X(4, 10, 8, 4, CfgThV1)
+    X(4, 10, 8, 4, CfgThV1)                         \
+    X(6, 10, 10, 10, CfgCubeV2L10)                  \
+    X(4, 10, 10, 10, CfgThV1L10)
 
 static bool desc_supported(const nepx_pair_model* m) {
 #define X(ns, a, b, c, T) \
@@ -572,7 +584,8 @@ extern "C" int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, co
 #define NEPX_NET_CASES(X)              \
     X(40, 64, 1, Net40x64, kNetTiles)  \
     X(60, 128, 1, Net60x128, 1)        \
-    X(47, 150, 3, Net47x150, 1)
+    X(47, 150, 3, Net47x150, 1)        \
+    X(121, 150, 3, Net121x150, 1)
 
 static bool net_supported(const nepx_net* n) {
 #define X(d, w, h, T, NT) if (n->d_in == d && n->width == w && n->n_hidden == h) return true;
@@ -733,4 +746,52 @@ extern "C" int nepx_fd_forces(const float* E_rows, int64_t P, float dx, float dt
                        (int)P, dx, dteta, clip, out);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Descriptors of arbitrary site clouds (training-set featurisation,
+// 05/NEP.py:171-219 get_g_fast and 04/Processors.py:48-88): one cloud per
+// lane, sites relative to the cloud centroid, optional per-site weights.
+template <class Cfg>
+__global__ __launch_bounds__(kBlock) void k_cloud_desc(const float* __restrict__ clouds, int64_t P, int nd,
+                                                       const float* __restrict__ w, float r_nep,
+                                                       float* __restrict__ G) {
+    for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < P; idx += (int64_t)gridDim.x * kBlock) {
+        const CloudSites sites(clouds + idx * 3 * (int64_t)nd, w, nd, 1.f / r_nep);
+        if constexpr (Cfg::ONE_PASS) {
+            float Gr[Cfg::D];
+            desc_forward_sites<Cfg>(sites, Gr);
+            store_row<Cfg>(G + idx * Cfg::D, Gr);
+        } else {
+            desc_forward_sites<Cfg>(sites, G + idx * Cfg::D);
+        }
+    }
+}
+
+#define NEPX_CLOUD_CASES(X)     \
+    X(11, 6, 4)                 \
+    X(10, 8, 4)                 \
+    X(17, 6, 6)                 \
+    X(10, 10, 10)
+
+extern "C" int nepx_cloud_descriptor(const float* clouds, int64_t n_clouds, int32_t n_sites, const float* site_w,
+                                     int32_t nrad, int32_t nang, int32_t lmax, float r_nep, float* G,
+                                     void* stream) {
+    NEPX_REQUIRE(n_clouds >= 0 && n_sites > 0, NEPX_EINVAL, "bad cloud count %lld or size %d",
+                 (long long)n_clouds, n_sites);
+    NEPX_REQUIRE(r_nep > 0.f, NEPX_EINVAL, "r_nep must be positive");
+    if (n_clouds == 0) return NEPX_OK;
+    NEPX_REQUIRE(clouds && G, NEPX_EINVAL, "null buffer");
+    const int grid = grid_for(n_clouds, kBlock);
+#define X(a, b, c)                                                                                     \
+    if (nrad == a && nang == b && lmax == c) {                                                         \
+        hipLaunchKernelGGL((k_cloud_desc<DescCfg<1, a, b, c>>), dim3(grid), dim3(kBlock), 0, as_stream(stream), \
+                           clouds, n_clouds, (int)n_sites, site_w, r_nep, G);                           \
+        NEPX_LAUNCH_CHECK();                                                                           \
+        return NEPX_OK;                                                                                \
+    }
+    NEPX_CLOUD_CASES(X)
+#undef X
+    set_error("no cloud descriptor kernel for hypers (%d, %d, %d)", nrad, nang, lmax);
+    return NEPX_EUNSUP;
 }

@@ -79,3 +79,25 @@ def pair_fused(model, net: EnergyNet, qf, pi, pj, pt, clip=0.0, n_dev=None, stre
     _lib.check(_lib.lib.nepx_pair_fused(C.byref(model), C.byref(cn), *a, _lib.ptr(n_dev), P, float(clip),
                                         _lib.ptr(out), _lib.stream_handle(stream)), "nepx_pair_fused")
     return out
+
+
+def cloud_descriptor(clouds, nrad, nang, lmax, r_nep, weights=None, stream=None):
+    """Descriptors (P, D) float32 of P site clouds (P, Nd, 3) float32 on the GPU, each taken about its
+    own centroid, with optional per-site weights (Nd,) (nepx_cloud_descriptor)."""
+    if clouds.dtype != torch.float32 or not clouds.is_cuda or clouds.dim() != 3 or clouds.shape[2] != 3:
+        raise TypeError("clouds must be a (P, Nd, 3) float32 CUDA tensor")
+    clouds = clouds.contiguous()
+    P, nd, _ = clouds.shape
+    D = n_descriptor(nrad, nang, lmax)
+    G = torch.empty((P, D), dtype=torch.float32, device=clouds.device)
+    if P == 0:
+        return G
+    w = None
+    if weights is not None:
+        w = torch.as_tensor(weights, dtype=torch.float32, device=clouds.device).contiguous()
+        if w.numel() != nd:
+            raise ValueError(f"{w.numel()} site weights for clouds of {nd} sites")
+    _lib.check(_lib.lib.nepx_cloud_descriptor(_lib.ptr(clouds), P, nd, _lib.ptr(w) if w is not None else None,
+                                              int(nrad), int(nang), int(lmax), float(r_nep), _lib.ptr(G),
+                                              _lib.stream_handle(stream)), "nepx_cloud_descriptor")
+    return G

@@ -194,6 +194,17 @@ int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* 
                     const int32_t* n_pairs_dev, int64_t max_pairs, float clip, float* out,
                     void* stream);
 
+/* Descriptors of arbitrary site clouds, for training-set featurisation.
+ * Replaces NEP.get_g_fast (05/NEP.py:171-219, 285-293) and the per-chunk loop of
+ * pts_to_nep (04/Processors.py:48-88, 05/Processors.py:46-92).
+ * clouds: n_clouds x n_sites x 3 floats (xyz interleaved), sites taken relative
+ * to each cloud's centroid; site_w: n_sites per-site weights (the type
+ * weights of NEP.set_pts_types) or NULL for 1; G: n_clouds x D floats with
+ * D = (nrad+1) + (nang+1)*lmax.  Hypers with a kernel: (11,6,4), (10,8,4),
+ * (17,6,6), (10,10,10); others return NEPX_EUNSUP. */
+int nepx_cloud_descriptor(const float* clouds, int64_t n_clouds, int32_t n_sites, const float* site_w,
+                          int32_t nrad, int32_t nang, int32_t lmax, float r_nep, float* G, void* stream);
+
 /* ---- neighbour list ---------------------------------------------------- */
 /* Rebuild the half list within r_list of each system when flags[0] != 0
  * (always when force != 0).  Replaces the host cKDTree query

@@ -226,13 +226,15 @@ def n_descriptor(nrad, nang, lmax):
     return (nrad + 1) + (nang + 1) * lmax
 
 
-def descriptor(clouds, r_nep, nrad, nang, lmax):
+def descriptor(clouds, r_nep, nrad, nang, lmax, weights=None):
     """NEP descriptor of each pair cloud, direct (all a, b) form (App. A.2).
 
-    g_rad[n] = sum_a f_n(a)                                  n = 0..nrad
-    g_ang[n*lmax + l-1] = sum_a sum_b f_n(a) f_n(b) P_l(cos_ab)  n = 0..nang, l = 1..lmax
+    g_rad[n] = sum_a w_a f_n(a)                                        n = 0..nrad
+    g_ang[n*lmax + l-1] = sum_a sum_b w_a f_n(a) w_b f_n(b) P_l(cos_ab)  n = 0..nang, l = 1..lmax
     The angular double sum includes a == b and both orders, as the live
     path does (DescriptorGenerator.py:376-411, SingleKernelDebug.py:23-73).
+    Per-site weights w (default 1) are the training-side type weights of
+    05/NEP.py:152-190 (chebo *= types before both sums).
     clouds (P, Nd, 3) float64 -> G (P, D) float64.
     """
     assert nang <= nrad
@@ -240,6 +242,8 @@ def descriptor(clouds, r_nep, nrad, nang, lmax):
     r = np.linalg.norm(c, axis=-1)                      # (P, Nd)
     u = c / r[..., None]
     f, _ = _cheb_basis(r / r_nep, nrad)                 # (nrad+1, P, Nd)
+    if weights is not None:
+        f = f * np.asarray(weights, dtype=np.float64)[None, None, :]
     g_rad = f.sum(axis=-1).T                            # (P, nrad+1)
     cos = np.einsum("pak,pbk->pab", u, u)
     cos = np.clip(cos, -1.0, 1.0)

@@ -13,7 +13,8 @@ from pairs_util import normwise_rel, oracle_clouds, random_pairs, rowwise_rel
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("tet10", (11, 6, 4)), ("cube8", (11, 6, 4)), ("cyl12", (17, 6, 6)), ("cube_v2", (10, 8, 4))]
+CASES = [("tet10", (11, 6, 4)), ("cube8", (11, 6, 4)), ("cyl12", (17, 6, 6)), ("cube_v2", (10, 8, 4)),
+         ("th_v1", (10, 10, 10))]
 
 
 def _inputs(gpu, shape_name, n_pairs=777, seed=1):
@@ -55,7 +56,8 @@ def test_empty_pair_set(gpu):
 
 @pytest.mark.parametrize("precision", ["f32", "f16x3"])
 @pytest.mark.parametrize("d_in,widths,act", [(40, [64, 64], "tanh"), (60, [128, 128], "tanh"),
-                                             (40, [64, 64], "relu"), (47, [150] * 4, "relu")])
+                                             (40, [64, 64], "relu"), (47, [150] * 4, "relu"),
+                                             (121, [150] * 4, "relu")])
 def test_ffnn_matches_oracle(gpu, d_in, widths, act, precision):
     import nepx
     from nepx import ops

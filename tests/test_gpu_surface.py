@@ -26,9 +26,9 @@ def _init():
     return GsdHandler(os.path.join(GOLD, "th_init_N200.gsd"))
 
 
-def _net(gpu, act="relu"):
+def _net(gpu, act="relu", D=47):
     import nepx
-    return nepx.EnergyNet.random(47, [150, 150, 150, 150], seed=21, act=act, en_min=-6.7, en_max=15.0, device=gpu)
+    return nepx.EnergyNet.random(D, [150, 150, 150, 150], seed=21, act=act, en_min=-6.7, en_max=15.0, device=gpu)
 
 
 def _kdtree_pairs(x, L, r):
@@ -36,12 +36,12 @@ def _kdtree_pairs(x, L, r):
     return tree.query_pairs(r=r, output_type="ndarray")
 
 
-def _dg_fe(gpu, h, net, fd=False, dx=1e-5):
+def _dg_fe(gpu, h, net, fd=False, dx=1e-5, hyp=HYP):
     from nepx.DescriptorGenerator import DescriptorGenerator
     from nepx.ForceEvaluator import ForceEvaluator
     dg = DescriptorGenerator()
     dg.setDevice(0)
-    dg.setHyperParameters([10, 8, 4, 4.5])
+    dg.setHyperParameters([*hyp, 4.5])
     dg.setBoxSize(h.getLx())
     dg.set_dx_dteta(dx, dx)
     dg.setShape("th_v1")
@@ -58,13 +58,17 @@ def _dg_fe(gpu, h, net, fd=False, dx=1e-5):
     return dg, fe
 
 
-def test_descriptor_generator_and_force_evaluator(gpu):
+@pytest.mark.parametrize("hyp", [HYP, (10, 10, 10)])
+def test_descriptor_generator_and_force_evaluator(gpu, hyp):
+    """(10, 10, 10) is the reference's documented run (sim_input.py:10: th_v1, --hypers 10 10 10 4.5,
+    a TheEnergyNet(150, 3, 121) checkpoint); D = 121 takes eight l-range passes per pair."""
+    import nepx
     h = _init()
     x = h.get_central_pos().astype(np.float64)
     q = h.getOrientations().astype(np.float64)
     L = h.getLx()
-    net = _net(gpu)
-    dg, fe = _dg_fe(gpu, h, net)
+    net = _net(gpu, D=nepx.n_descriptor(*hyp))
+    dg, fe = _dg_fe(gpu, h, net, hyp=hyp)
     nlist = torch.from_numpy(_kdtree_pairs(x, L, 5.75 + 2.0)).to(gpu)
     G, pp, P = dg.generate_nep_descriptors(torch.from_numpy(x).to(gpu), torch.from_numpy(q).to(gpu), nlist)
     want = O.brute_pairs(x, L, 5.75)
@@ -72,14 +76,14 @@ def test_descriptor_generator_and_force_evaluator(gpu):
     sh = O.shape_table()["th_v1"]
     q32 = q.astype(np.float32).astype(np.float64)
     clouds, _ = O.pair_clouds(x, q32, want, sh["sites"], L)
-    G64 = O.descriptor(clouds, 4.5, *HYP)
+    G64 = O.descriptor(clouds, 4.5, *hyp)
     eG = np.max(np.abs(G.cpu().numpy() - G64), axis=1) / np.max(np.abs(G64), axis=1)
     F, T = fe.evaluate_interactions(G, pp, P)
-    F64, T64, _, r = O.system_forces(x, q32, sh, L, HYP, net.layers_f64(), "relu", -6.7, 15.0)
+    F64, T64, _, r = O.system_forces(x, q32, sh, L, hyp, net.layers_f64(), "relu", -6.7, 15.0)
     eF = np.abs(F.cpu().numpy() - F64).max() / np.abs(F64).max()
     eT = np.abs(T.cpu().numpy() - T64).max() / np.abs(T64).max()
     eE = np.abs(fe.e0.cpu().numpy()[:, 0] - r["E"]).max() / np.abs(r["E"]).max()
-    print(f"th_v1 N=200: {P} pairs, G {eG.max():.2e} E {eE:.2e} F {eF:.2e} T {eT:.2e}")
+    print(f"th_v1 N=200 hypers {hyp}: {P} pairs, G {eG.max():.2e} E {eE:.2e} F {eF:.2e} T {eT:.2e}")
     assert eG.max() < 1e-5 and eE < 1e-5 and eF < 1e-4 and eT < 1e-4
 
 

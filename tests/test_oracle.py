@@ -363,3 +363,17 @@ def test_thermostat_fixed_point_and_direction():
     assert np.isclose(u["theta"], math.exp(-0.5 * 0.2 * 0.01))
     hot = O.thermo_update(th, 2 * K_eq, 0.5 * K_eq, N_dof, N_dof, 0.01, 1.0, kT)
     assert hot["xi"] > th["xi"] and hot["xi_r"] < th["xi_r"]
+
+
+def test_descriptor_weights_equal_site_multiplicity():
+    """Integer per-site weights k_a give the descriptor of the cloud with site a repeated k_a times
+    (the radial sum and both angular sums count it k_a times).  The clouds are point-symmetric with
+    symmetric weights, so the repeated cloud keeps the centroid the weighted one is taken about."""
+    rng = np.random.default_rng(11)
+    X2 = rng.normal(0, 1, (5, 3, 3))
+    sym = np.concatenate([X2, -X2], axis=1)
+    k = np.array([2, 1, 3, 2, 1, 3])
+    G_w = O.descriptor(sym, 3.0, 5, 4, 3, weights=k.astype(float))
+    G_r = O.descriptor(np.repeat(sym, k, axis=1), 3.0, 5, 4, 3)
+    assert np.allclose(G_w, G_r, rtol=1e-12, atol=1e-12)
+    assert not np.allclose(G_w, O.descriptor(sym, 3.0, 5, 4, 3))
