@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "nepx", "libnepx.so")
-SOURCES = ["nepx_api.hip", "nepx_pair.hip", "nepx_nlist.hip", "nepx_integrate.hip"]
+SOURCES = ["nepx_api.hip", "nepx_pair.hip", "nepx_nlist.hip", "nepx_integrate.hip", "nepx_bead.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

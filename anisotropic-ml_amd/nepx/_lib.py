@@ -99,6 +99,8 @@ def _load():
         "nepx_pair_fused": ([C.POINTER(PairModel), C.POINTER(Net), vp, vp, vp, vp, vp, C.c_int64, C.c_float,
                              vp, vp], C.c_int),
         "nepx_pair_fused_supported": ([C.POINTER(PairModel), C.POINTER(Net)], C.c_int),
+        "nepx_bead_pair_forces": ([vp, C.c_int32, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp, vp, C.c_int64,
+                                   C.c_float, vp, vp], C.c_int),
         "nepx_cloud_descriptor": ([vp, C.c_int64, C.c_int32, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                    vp, vp], C.c_int),
         "nepx_nlist_build": ([C.POINTER(State), C.POINTER(NList), C.c_int32, vp], C.c_int),

@@ -56,7 +56,7 @@ class Arena:
     """One or more systems sharing a pair model and energy network, resident on one GPU."""
 
     def __init__(self, specs, shape: Shape, hypers, net: EnergyNet, device="cuda", clip=100.0,
-                 list_cap=None, r_nep=None):
+                 list_cap=None, r_nep=None, potential=None):
         if not torch.cuda.is_available():
             raise _lib.NepxError("nepx needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device(device)
@@ -67,6 +67,8 @@ class Arena:
         if net is not None and net.d_in != self.D:
             raise ValueError(f"network input {net.d_in} != descriptor size {self.D}")
         self.net = net
+        # a pair potential in place of the NEP model (label.BeadPotential); None = descriptor + network
+        self.potential = potential
         self.model = make_pair_model(shape, self.hypers, r_nep)
         self.cnet = net.cstruct() if net is not None else None
         self.clip = float(clip)
@@ -274,6 +276,9 @@ class Arena:
         np_dev = C.c_void_p(self.lt["flags"].data_ptr() + 4 * _lib.FLAG_P)
         args = (_lib.ptr(self.t["qf"]), _lib.ptr(self.lt["pair_i"]), _lib.ptr(self.lt["pair_j"]),
                 _lib.ptr(self.lt["pair_t"]), np_dev, self.cap)
+        if self.potential is not None:
+            self.potential.pair_forces(*args, _lib.ptr(self.pair_out), self.clip, stream)
+            return
         self.prepare_pair()
         if not self.split:
             _lib.check(_lib.lib.nepx_pair_fused(C.byref(self.model), C.byref(self.cnet), *args, self.clip,
@@ -290,6 +295,8 @@ class Arena:
 
     def prepare_pair(self):
         """Choose fused or split pair kernels and allocate the split path's buffers (no launches)."""
+        if self.potential is not None:
+            return
         if getattr(self, "split", None) is None:
             self.split = not _lib.lib.nepx_pair_fused_supported(C.byref(self.model), C.byref(self.cnet))
         if self.split and (getattr(self, "_G", None) is None or self._G.numel() < self.cap * self.D):

@@ -205,6 +205,20 @@ int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* 
 int nepx_cloud_descriptor(const float* clouds, int64_t n_clouds, int32_t n_sites, const float* site_w,
                           int32_t nrad, int32_t nang, int32_t lmax, float r_nep, float* G, void* stream);
 
+/* ---- bead (label) potential -------------------------------------------- */
+/* Soft bead-bead repulsion between rigid bodies, the label potential of the
+ * reference's data generator (01/SamplerUtils.py:555-631, calc_en_unred):
+ * E_ij = sum_{a in i, b in j} eps ((r0 - d_ab)/sigma)^2 for d_ab < r0, with
+ * body-frame beads (n_beads x 3 floats, the same table for both bodies) placed
+ * by the pair's orientations and minimum-image offset.  Pair inputs and the
+ * per-pair output {F_i, E}{tau_i, 0}{tau_j, 0} are those of nepx_pair_fused, so
+ * nepx_forces_gather and the integrator take the result unchanged.
+ * The reference values are eps = 5, sigma = 1.1, r0 = 2^(1/6). */
+#define NEPX_MAX_BEADS 256
+int nepx_bead_pair_forces(const float* beads, int32_t n_beads, float eps, float sigma, float r0,
+                          const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                          const int32_t* n_pairs_dev, int64_t max_pairs, float clip, float* out, void* stream);
+
 /* ---- neighbour list ---------------------------------------------------- */
 /* Rebuild the half list within r_list of each system when flags[0] != 0
  * (always when force != 0).  Replaces the host cKDTree query

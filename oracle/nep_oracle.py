@@ -588,3 +588,72 @@ def nvt_step(state, F, T, dt, kT, tau, L, moi, thermo):
     Kt = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
     Kr = float(np.sum(rot_ke(q, p, moi)))
     return dict(x=x, v=v, q=q, p=p, m=m), Kt, Kr
+
+
+# ---------------------------------------------------------------------------
+# Label potential: soft bead-bead repulsion (01/SamplerUtils.py:555-631)
+# ---------------------------------------------------------------------------
+
+BEAD_EPS, BEAD_SIGMA, BEAD_R0 = 5.0, 1.1, 2.0 ** (1.0 / 6.0)
+
+
+def cube_structure_beads():
+    """152 surface points of a 6x6x6 grid of spacing 0.7 centred at the origin (the bead layout of
+    01/cube_structure.gsd, in its order: z fastest, then x, then y)."""
+    g = -1.75 + 0.7 * np.arange(6)
+    pts = [(x, y, z) for y in g for x in g for z in g if max(abs(x), abs(y), abs(z)) > 1.75 - 1e-9]
+    return np.array(pts, dtype=np.float64)
+
+
+def bead_pair_forces(X, Q, pairs, beads, L, eps=BEAD_EPS, sigma=BEAD_SIGMA, r0=BEAD_R0):
+    """Per pair: E = sum_ab eps ((r0 - d_ab)/sigma)^2 [d_ab < r0] over body-frame beads placed by
+    quaternions Q and minimum-image offsets; F on i, torques on i and j (FP64, direct double loop
+    over bead pairs, 01/SamplerUtils.py:615-628)."""
+    pairs = np.asarray(pairs)
+    i, j = pairs[:, 0], pairs[:, 1]
+    t = min_image(X[j] - X[i], L)
+    ai = quat_rotate(np.repeat(Q[i], len(beads), axis=0), np.tile(beads, (len(i), 1))).reshape(len(i), -1, 3)
+    aj = quat_rotate(np.repeat(Q[j], len(beads), axis=0), np.tile(beads, (len(j), 1))).reshape(len(j), -1, 3)
+    d = (t[:, None, None, :] + aj[:, None, :, :]) - ai[:, :, None, :]      # (P, a, b, 3): p_b - p_a
+    r = np.linalg.norm(d, axis=-1)
+    on = r < r0
+    w = np.where(on, r0 - r, 0.0)
+    E = (eps * w * w / sigma ** 2).sum(axis=(1, 2))
+    s = np.where(on, -2.0 * eps * w / (sigma ** 2 * np.where(on, r, 1.0)), 0.0)
+    g = s[..., None] * d                                                    # dE/dp_b
+    F = g.sum(axis=(1, 2))
+    Ti = np.cross(ai[:, :, None, :], g).sum(axis=(1, 2))
+    Tj = -np.cross(aj[:, None, :, :], g).sum(axis=(1, 2))
+    return dict(E=E, F=F, Ti=Ti, Tj=Tj)
+
+
+def euler_zxz(angles):
+    """Rotation matrices of the unreduced pair table (01/SamplerUtils.py:589-605), angles (P, 3)."""
+    s1, c1 = np.sin(angles[:, 0]), np.cos(angles[:, 0])
+    s2, c2 = np.sin(angles[:, 1]), np.cos(angles[:, 1])
+    s3, c3 = np.sin(angles[:, 2]), np.cos(angles[:, 2])
+    R = np.empty((len(angles), 3, 3))
+    R[:, 0, 0] = c1 * c3 - c2 * s1 * s3
+    R[:, 0, 1] = -c1 * s3 - c2 * c3 * s1
+    R[:, 0, 2] = s1 * s2
+    R[:, 1, 0] = c3 * s1 + c1 * c2 * s3
+    R[:, 1, 1] = -s1 * s3 + c1 * c2 * c3
+    R[:, 1, 2] = -c1 * s2
+    R[:, 2, 0] = s2 * s3
+    R[:, 2, 1] = c3 * s2
+    R[:, 2, 2] = c2
+    return R
+
+
+def calc_en_unred(unred, beads, eps=BEAD_EPS, sigma=BEAD_SIGMA, r0=BEAD_R0):
+    """Energy of each unreduced table row (01/SamplerUtils.py:555-631): body 1 at the origin
+    unrotated, body 2 rotated by the ZXZ angles unred[:, 0:3] and moved to spherical
+    (r, azimuth, polar) = unred[:, 3:6]."""
+    unred = np.asarray(unred, dtype=np.float64)
+    st = np.sin(unred[:, 5])
+    t = np.stack([unred[:, 3] * st * np.cos(unred[:, 4]), unred[:, 3] * st * np.sin(unred[:, 4]),
+                  unred[:, 3] * np.cos(unred[:, 5])], axis=1)
+    p2 = np.einsum("pij,bj->pbi", euler_zxz(unred[:, :3]), beads) + t[:, None, :]
+    d = np.linalg.norm(p2[:, None, :, :] - beads[None, :, None, :], axis=-1)
+    w = np.where(d < r0, (r0 - d) / sigma, 0.0)
+    return (eps * w * w).sum(axis=(1, 2))

@@ -296,3 +296,17 @@ def test_coordinate_descent_finds_separable_minimum(tmp_path):
     at = tune.Autotuner(str(tmp_path / "t.json"))
     at._store("k", best)
     assert at._load()["k"]["gather_lanes"] == 8
+
+
+def test_label_host_helpers_match_oracle():
+    from scipy.spatial.transform import Rotation
+    from nepx import label
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-3.2, 3.2, (50, 3))
+    R = label.euler_zxz(a)
+    assert np.abs(R - O.euler_zxz(a)).max() < 1e-14
+    q = label.matrix_to_quat(R)
+    qx = Rotation.from_matrix(R).as_quat()                       # scalar last, sign free
+    ref = np.concatenate([qx[:, 3:], qx[:, :3]], axis=1)
+    assert np.abs(np.abs(np.sum(q * ref, axis=1)) - 1.0).max() < 1e-12
+    assert np.abs(label.cube_structure_beads() - O.cube_structure_beads()).max() == 0.0

@@ -6,6 +6,7 @@ cKDTree, torch nn.Linear + autograd, finite differences, the addition
 theorem, symmetry, or a closed-form solution).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -377,3 +378,82 @@ def test_descriptor_weights_equal_site_multiplicity():
     G_r = O.descriptor(np.repeat(sym, k, axis=1), 3.0, 5, 4, 3)
     assert np.allclose(G_w, G_r, rtol=1e-12, atol=1e-12)
     assert not np.allclose(G_w, O.descriptor(sym, 3.0, 5, 4, 3))
+
+
+# ---- label potential (01/SamplerUtils.py:555-631) ----------------------------
+
+def test_cube_beads_equal_reference_structure_file():
+    from nepx import gsd
+    x = gsd.read_frame(os.path.join(os.path.dirname(__file__), "golden", "cube_structure.gsd"))["particles/position"]
+    assert np.abs(O.cube_structure_beads() - x).max() < 1e-6
+
+
+def test_euler_matrices_are_intrinsic_zxz():
+    from scipy.spatial.transform import Rotation
+    a = np.random.default_rng(0).uniform(-3.2, 3.2, (20, 3))
+    assert np.abs(Rotation.from_euler("ZXZ", a).as_matrix() - O.euler_zxz(a)).max() < 1e-12
+
+
+def test_bead_energy_closed_form_and_symmetry():
+    one = np.zeros((1, 3))
+    X = np.array([[0.0, 0.0, 0.0], [0.8, 0.3, -0.2]])
+    Q = np.array([[1.0, 0, 0, 0], [1.0, 0, 0, 0]])
+    r = O.bead_pair_forces(X, Q, np.array([[0, 1]]), one, 50.0)
+    d = np.linalg.norm(X[1])
+    assert np.isclose(r["E"][0], 5.0 * ((2 ** (1 / 6) - d) / 1.1) ** 2, rtol=1e-14)
+    far = O.bead_pair_forces(np.array([[0.0, 0, 0], [1.2, 0, 0]]), Q, np.array([[0, 1]]), one, 50.0)
+    assert far["E"][0] == 0.0 and np.all(far["F"] == 0.0)
+    beads = O.cube_structure_beads()
+    rng = np.random.default_rng(2)
+    X = np.array([[0.0, 0, 0], [5.0, 1.0, -0.5]])
+    Q = O.quat_mul(np.eye(4)[[0, 0]], rng.normal(size=(2, 4)))
+    Q /= np.linalg.norm(Q, axis=1, keepdims=True)
+    a = O.bead_pair_forces(X, Q, np.array([[0, 1]]), beads, 50.0)
+    b = O.bead_pair_forces(X[::-1].copy(), Q[::-1].copy(), np.array([[0, 1]]), beads, 50.0)
+    assert a["E"][0] > 0 and np.isclose(a["E"][0], b["E"][0], rtol=1e-12)
+    assert np.allclose(a["F"], -b["F"], rtol=1e-10, atol=1e-12)
+    assert np.allclose(a["Ti"], b["Tj"], rtol=1e-10, atol=1e-12)
+
+
+def test_bead_forces_central_fd():
+    beads = O.cube_structure_beads()
+    rng = np.random.default_rng(3)
+    X = np.array([[0.0, 0, 0], [4.6, 1.3, -0.9]])
+    Q = rng.normal(size=(2, 4))
+    Q /= np.linalg.norm(Q, axis=1, keepdims=True)
+    pr = np.array([[0, 1]])
+    r = O.bead_pair_forces(X, Q, pr, beads, 50.0)
+    assert r["E"][0] > 0
+    h = 1e-6
+    for k in range(3):
+        Xp, Xm = X.copy(), X.copy()
+        Xp[0, k] += h
+        Xm[0, k] -= h
+        fd = -(O.bead_pair_forces(Xp, Q, pr, beads, 50.0)["E"][0] - O.bead_pair_forces(Xm, Q, pr, beads, 50.0)["E"][0]) / (2 * h)
+        assert abs(fd - r["F"][0, k]) < 1e-5 * np.abs(r["F"]).max()
+        for body, key in ((0, "Ti"), (1, "Tj")):
+            dq = np.zeros(4)
+            dq[0], dq[1 + k] = np.cos(h / 2), np.sin(h / 2)
+            Qp, Qm = Q.copy(), Q.copy()
+            Qp[body] = O.quat_mul(dq, Q[body])
+            Qm[body] = O.quat_mul(dq * np.array([1, -1, -1, -1]), Q[body])
+            fd = -(O.bead_pair_forces(X, Qp, pr, beads, 50.0)["E"][0] - O.bead_pair_forces(X, Qm, pr, beads, 50.0)["E"][0]) / (2 * h)
+            assert abs(fd - r[key][0, k]) < 1e-5 * np.abs(r[key]).max()
+
+
+def test_calc_en_unred_equals_pair_form():
+    """The unreduced-table energy equals the pair form with body 2's quaternion from the ZXZ matrix."""
+    from scipy.spatial.transform import Rotation
+    beads = O.cube_structure_beads()
+    rng = np.random.default_rng(4)
+    u = np.column_stack([rng.uniform(-3.14, 3.14, 6), rng.uniform(0, 3.14, 6), rng.uniform(-3.14, 3.14, 6),
+                         rng.uniform(4.3, 6.0, 6), rng.uniform(-3.14, 3.14, 6), rng.uniform(0, 3.14, 6)])
+    E = O.calc_en_unred(u, beads)
+    assert (E > 0).sum() >= 4
+    for k in range(6):
+        qx = Rotation.from_matrix(O.euler_zxz(u[k:k + 1, :3])).as_quat()[0]      # scalar last
+        q = np.array([[1.0, 0, 0, 0], [qx[3], qx[0], qx[1], qx[2]]])
+        st = np.sin(u[k, 5])
+        t = u[k, 3] * np.array([st * np.cos(u[k, 4]), st * np.sin(u[k, 4]), np.cos(u[k, 5])])
+        r = O.bead_pair_forces(np.array([[0.0, 0, 0], t]), q, np.array([[0, 1]]), beads, 100.0)
+        assert np.isclose(r["E"][0], E[k], rtol=1e-12)
