@@ -20,6 +20,8 @@ reference source text (never imported):
 * NVT integrator            nep_simulator/NVT.py:17-129,
                             nep_simulator/cudakernels/IntegrateStepOneKernels.py:3-251,
                             nep_simulator/cudakernels/IntegrateStepTwoKernels.py:2-130
+* site-cloud type weights   05/NEP.py:152-190 (descriptor(..., weights=))
+* label potential           01/SamplerUtils.py:555-631 (bead_pair_forces, calc_en_unred)
 
 Pinning (each function is checked in tests/test_oracle.py against something
 other than its own formula): scipy Rotation (placement), scipy eval_chebyt /
@@ -27,8 +29,11 @@ eval_legendre (descriptor bases), the spherical-harmonic addition theorem
 (angular descriptor), brute force vs scipy cKDTree (pair sets), torch
 nn.Linear in float64 (FFNN), torch autograd + central finite differences
 (adjoint forces and torques), closed-form free-particle / spherical-top
-motion and time reversibility (integrator).  No reference-held golden
-vectors exist for this path (the reference has no tests).
+motion and time reversibility (integrator), site multiplicity (type
+weights), scipy Euler angles, a closed form and finite differences (label
+potential).  No reference-held golden vectors exist for this path (the
+reference has no tests); its data files (pair clouds, initial configuration,
+bead table) are used as inputs under tests/golden/.
 """
 import math
 

@@ -1,4 +1,4 @@
-"""The reference algorithm of one MD step in FP64 numpy -- TEST / BASELINE INFRASTRUCTURE ONLY.
+"""The reference algorithm of one MD step in FP64 numpy -- TEST AND CPU-BASELINE INFRASTRUCTURE ONLY.
 
 Used by bench.py's cpu_baseline leg ("port": the reference's own algorithm,
 restated) and by tests.  Follows nep_simulator/Sim.py:257-285:
