@@ -117,6 +117,15 @@ struct LRange {
     static constexpr int N = (L1 + 1) * (L1 + 1) - L0 * L0;
 };
 
+// First l of the last pass.
+constexpr int last_pass_start(int nang, int lmax) {
+    int l0 = 1;
+    while (pass_end(nang, lmax, l0) < lmax) l0 = pass_end(nang, lmax, l0) + 1;
+    return l0;
+}
+template <class Cfg>
+using LastPass = LRange<last_pass_start(Cfg::NANG, Cfg::LMAX), Cfg::LMAX>;
+
 // Calls fn(LRange<...>{}, first) for each pass of Cfg's l range, in order.
 template <class Cfg, int L0 = 1, class Fn>
 __device__ __forceinline__ void for_passes(Fn&& fn) {

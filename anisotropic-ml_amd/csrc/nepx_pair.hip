@@ -288,20 +288,34 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
         float4 qi = make_float4(1.f, 0.f, 0.f, 0.f), qj = qi, t = make_float4(1.f, 0.f, 0.f, 0.f);
         if (valid) load_pair(qf, pi, pj, pt, idx, qi, qj, t);
         const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
-        // one-pass shapes keep the forward accumulators for the adjoint
-        float A[Cfg::ONE_PASS ? Cfg::NANG + 1 : 1][Cfg::ONE_PASS ? Cfg::NLM : 1];
+        // one-pass shapes keep the forward accumulators in registers for the adjoint; multi-pass
+        // shapes recompute every range there (keeping the last range's through the network phase
+        // spills: measured slower on the 262,144-cylinder step, 14.8 vs 12.9 ms)
+        using LRL = LastPass<Cfg>;
+        constexpr bool KEEP = Cfg::ONE_PASS;
+        float A[KEEP ? Cfg::NANG + 1 : 1][KEEP ? LRL::N : 1];
         {
             if constexpr (Cfg::ONE_PASS) {
                 float Gr[Cfg::D];
                 float grad[Cfg::NRAD + 1];
-                desc_accumulate<Cfg, LRange<1, Cfg::LMAX>, true>(m, g, grad, A);
-                desc_finish<Cfg, LRange<1, Cfg::LMAX>>(A, Gr);
+                desc_accumulate<Cfg, LRL, true>(m, g, grad, A);
+                desc_finish<Cfg, LRL>(A, Gr);
 #pragma unroll
                 for (int n = 0; n <= Cfg::NRAD; ++n) Gr[n] = grad[n];
 #pragma unroll
                 for (int k = 0; k < Cfg::D; ++k) myrow[k] = valid ? Gr[k] : 0.f;
             } else {  // several passes: each writes its entries straight into the LDS row
-                desc_forward<Cfg>(m, g, myrow);
+                float grad[Cfg::NRAD + 1];
+                for_passes<Cfg>([&](auto range, auto first) {
+                    using LR = decltype(range);
+                    float Ap[Cfg::NANG + 1][LR::N];
+                    desc_accumulate<Cfg, LR, decltype(first)::value>(m, g, grad, Ap);
+                    desc_finish<Cfg, LR>(Ap, myrow);
+                    if constexpr (decltype(first)::value) {
+#pragma unroll
+                        for (int n = 0; n <= Cfg::NRAD; ++n) myrow[n] = grad[n];
+                    }
+                });
                 if (!valid)
                     for (int k = 0; k < Cfg::D; ++k) myrow[k] = 0.f;
             }
@@ -331,8 +345,17 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
 #pragma unroll
             for (int k = 0; k < Cfg::D; ++k) dG[k] = myrow[k];
             AdjAcc acc;
-            if constexpr (Cfg::ONE_PASS) desc_adjoint<Cfg, LRange<1, Cfg::LMAX>, true>(m, g, dG, A, acc);
-            else adjoint_all<Cfg>(m, g, dG, acc);
+            for_passes<Cfg>([&](auto range, auto first) {
+                using LR = decltype(range);
+                if constexpr (KEEP) {
+                    desc_adjoint<Cfg, LR, decltype(first)::value>(m, g, dG, A, acc);
+                } else {
+                    float grad[Cfg::NRAD + 1];
+                    float Ap[Cfg::NANG + 1][LR::N];
+                    desc_accumulate<Cfg, LR, false>(m, g, grad, Ap);
+                    desc_adjoint<Cfg, LR, decltype(first)::value>(m, g, dG, Ap, acc);
+                }
+            });
             store_out(out, idx, adj_finish<Cfg>(acc), etile[threadIdx.x], clip);
         }
         __syncthreads();
