@@ -441,9 +441,18 @@ def same_work_plain(jobs, costs, sample_scale=1.0):
     sum_j steps_done_j * ms_per_step_j (over the jobs that have a cost, times sample_scale)."""
     ms = sum(j.steps_done * costs[j.jid] for j in jobs if j.jid in costs) * sample_scale
     work = sum(j.n * j.steps_done for j in jobs)
-    return {"particle_steps": int(work), "seconds": ms * 1e-3,
-            "particle_steps_per_s": work / (ms * 1e-3) if ms > 0 else float("nan"),
-            "jobs": len(jobs), "jobs_timed": sum(1 for j in jobs if j.jid in costs)}
+    timed = [j for j in jobs if j.jid in costs and j.steps_done > 0]
+    out = {"particle_steps": int(work), "seconds": ms * 1e-3,
+           "particle_steps_per_s": work / (ms * 1e-3) if ms > 0 else float("nan"),
+           "jobs": len(jobs), "jobs_timed": len(timed)}
+    if timed:   # per-step latency of the plain step, each job weighted by its steps
+        lat = np.array([costs[j.jid] for j in timed])
+        w = np.array([j.steps_done for j in timed], dtype=np.float64)
+        order = np.argsort(lat)
+        cw = np.cumsum(w[order]) / w.sum()
+        out["step_ms_p50"] = float(lat[order][np.searchsorted(cw, 0.5)])
+        out["step_ms_p99"] = float(lat[order][min(np.searchsorted(cw, 0.99), len(lat) - 1)])
+    return out
 
 
 def poisson_jobs(rate, duration, seed, shapes=("cube8", "tet10"), n_range=(128, 65536),

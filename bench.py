@@ -110,7 +110,8 @@ def serve_bench(a):
     for name in a.serve_shapes.split(","):
         sh = nepx.get_shape(name)
         classes[name] = (sh, sh.hypers, nets[nepx.n_descriptor(*sh.hypers)])
-    jobs = poisson_jobs(rate=20.0, duration=a.serve_arrivals, seed=2025, shapes=tuple(classes))
+    jobs = poisson_jobs(rate=20.0, duration=a.serve_arrivals, seed=2025, shapes=tuple(classes),
+                        n_range=(128 if a.serve_n_max > 128 else 16, a.serve_n_max))
     plain_jobs = copy.deepcopy(jobs)
     def log(t, resident, done):
         print(f"[serve] t={t:.0f}s resident={resident} finished={done}", file=sys.stderr, flush=True)
@@ -134,7 +135,7 @@ def serve_bench(a):
         "served": served,
         "plain_same_work": plain_same,
         "config": {"workload": f"Poisson 20 jobs/s for {a.serve_arrivals} s of arrivals ({len(jobs)} jobs), "
-                               f"{'/'.join(classes)}, N log-uniform [128, 65536], steps log-uniform [1e3, 1e5], "
+                               f"{'/'.join(classes)}, N log-uniform [128, {a.serve_n_max}], steps log-uniform [1e3, 1e5], "
                                "kT U[0.5, 2], phi U[0.2, 0.45], FFNN D->64->64->1 (D=40) or D->128->128->1 "
                                "(D=60) tanh",
                    "window_s": a.serve_seconds, "capacity_per_class": a.serve_capacity, "chunk": a.serve_chunk,
@@ -227,6 +228,7 @@ def main():
     ap.add_argument("--serve-capacity", type=int, default=262144, help="serve workload: particles per class arena")
     ap.add_argument("--serve-chunk", type=int, default=20, help="serve workload: steps per scheduling chunk")
     ap.add_argument("--serve-shapes", default="cube8,tet10,cyl12", help="serve workload: job classes")
+    ap.add_argument("--serve-n-max", type=int, default=65536, help="serve workload: largest job size")
     ap.add_argument("--serve-plain-jobs", type=int, default=96,
                     help="serve workload: jobs timed alone for the same-work plain comparison (uniform sample)")
     ap.add_argument("--serve-plain-window", type=int, default=0,
