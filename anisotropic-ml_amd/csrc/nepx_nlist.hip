@@ -357,15 +357,42 @@ __global__ void k_rows_maxlen(nepx_nlist nl, int n) {
 }
 
 // ---- per-step filter and compaction ---------------------------------------
-__device__ __forceinline__ bool slot_active(const nepx_state& st, const nepx_nlist& nl, int s, int& i, int& j,
-                                            double& tx, double& ty, double& tz) {
-    i = nl.slot_row[s];
-    j = nl.nbr[s];
-    const int sy = st.sys[i];
-    if (sy < 0 || sy != st.sys[j]) return false;   // slots left over from a freed system
-    const nepx_sysparam& sp = st.sp[sy];
-    if (!sp.active) return false;
-    return pair_d2(st, sp.L, i, j, tx, ty, tz) < __dmul_rn(sp.rc, sp.rc);
+// Each thread evaluates kSlotsPerThread list slots (stride kNB within the
+// block's kSlotsPerBlock), issuing every load of all its slots before the
+// predicates so the gathers overlap.
+constexpr int kSlotsPerThread = kSlotsPerBlock / kNB;
+
+struct SlotEval {
+    bool act[kSlotsPerThread];
+    int i[kSlotsPerThread], j[kSlotsPerThread];
+    double tx[kSlotsPerThread], ty[kSlotsPerThread], tz[kSlotsPerThread];
+};
+
+__device__ __forceinline__ void eval_slots(const nepx_state& st, const nepx_nlist& nl, int base, int PL,
+                                           SlotEval& r) {
+    bool in[kSlotsPerThread];
+    int si[kSlotsPerThread], sj[kSlotsPerThread];
+#pragma unroll
+    for (int k = 0; k < kSlotsPerThread; ++k) {
+        const int s = base + k * kNB + threadIdx.x;
+        in[k] = s < PL;
+        r.i[k] = in[k] ? nl.slot_row[s] : 0;
+        r.j[k] = in[k] ? nl.nbr[s] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kSlotsPerThread; ++k) {
+        si[k] = st.sys[r.i[k]];
+        sj[k] = st.sys[r.j[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kSlotsPerThread; ++k) {
+        // slots left over from a freed system, or of an inactive one, are not pairs
+        const bool same = in[k] && si[k] >= 0 && si[k] == sj[k];
+        const nepx_sysparam& sp = st.sp[same ? si[k] : 0];
+        const bool live_sys = same && sp.active;
+        const double d2 = pair_d2(st, sp.L, r.i[k], r.j[k], r.tx[k], r.ty[k], r.tz[k]);
+        r.act[k] = live_sys && d2 < __dmul_rn(sp.rc, sp.rc);
+    }
 }
 
 __global__ __launch_bounds__(kNB) void k_filter_count(nepx_state st, nepx_nlist nl) {
@@ -375,13 +402,11 @@ __global__ __launch_bounds__(kNB) void k_filter_count(nepx_state st, nepx_nlist 
         if (threadIdx.x == 0) nl.blk_count[blockIdx.x] = 0;
         return;
     }
+    SlotEval r;
+    eval_slots(st, nl, base, PL, r);
     int cnt = 0;
-    for (int k = 0; k < kSlotsPerBlock / kNB; ++k) {
-        const int s = base + k * kNB + threadIdx.x;
-        int i, j;
-        double tx, ty, tz;
-        if (s < PL && slot_active(st, nl, s, i, j, tx, ty, tz)) ++cnt;
-    }
+#pragma unroll
+    for (int k = 0; k < kSlotsPerThread; ++k) cnt += r.act[k] ? 1 : 0;
     __shared__ int red[kNB];
     red[threadIdx.x] = cnt;
     __syncthreads();
@@ -407,35 +432,41 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
     const int PL = nl.flags[F_PL];
     const int base = blockIdx.x * kSlotsPerBlock;
     if (base >= PL || nl.flags[F_OVERFLOW]) return;
-    __shared__ int wsum[kNB / 64];
-    int out = nl.blk_count[blockIdx.x];
+    __shared__ int wsum[kSlotsPerThread][kNB / 64];
+    SlotEval r;
+    eval_slots(st, nl, base, PL, r);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int k = 0; k < kSlotsPerBlock / kNB; ++k) {
+    uint64_t below[kSlotsPerThread];
+#pragma unroll
+    for (int k = 0; k < kSlotsPerThread; ++k) {
+        const uint64_t mask = __ballot(r.act[k]);
+        below[k] = mask & ((1ull << lane) - 1ull);
+        if (lane == 0) wsum[k][wv] = __popcll(mask);
+    }
+    __syncthreads();
+    int out = nl.blk_count[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kSlotsPerThread; ++k) {
         const int s = base + k * kNB + threadIdx.x;
-        int i = 0, j = 0;
-        double tx = 0, ty = 0, tz = 0;
-        const bool act = s < PL && slot_active(st, nl, s, i, j, tx, ty, tz);
-        const uint64_t mask = __ballot(act);
-        const int before = __popcll(mask & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wv] = __popcll(mask);
-        __syncthreads();
-        int woff = 0;
-        for (int w = 0; w < wv; ++w) woff += wsum[w];
-        int tot = 0;
-        for (int w = 0; w < kNB / 64; ++w) tot += wsum[w];
+        int woff = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kNB / 64; ++w) {
+            woff += w < wv ? wsum[k][w] : 0;
+            tot += wsum[k][w];
+        }
         if (s < PL) {
-            if (act) {
-                const int c = out + woff + before;
+            if (r.act[k]) {
+                const int c = out + woff + __popcll(below[k]);
                 nl.cidx[s] = c;
-                nl.pair_i[c] = i;
-                nl.pair_j[c] = j;
-                reinterpret_cast<float4*>(nl.pair_t)[c] = make_float4((float)tx, (float)ty, (float)tz, 0.f);
+                nl.pair_i[c] = r.i[k];
+                nl.pair_j[c] = r.j[k];
+                reinterpret_cast<float4*>(nl.pair_t)[c] =
+                    make_float4((float)r.tx[k], (float)r.ty[k], (float)r.tz[k], 0.f);
             } else {
                 nl.cidx[s] = -1;
             }
         }
         out += tot;
-        __syncthreads();
     }
 }
 
