@@ -8,9 +8,6 @@ torques in the per-pair layout of the NEP kernels, so the gather and the integra
 to validate learned forces against the potential the labels came from.  calc_en_unred labels rows
 of the unreduced pair table (ZXZ Euler angles of body 2, then its spherical position) on the GPU.
 """
-import ctypes as C
-import math
-
 import numpy as np
 import torch
 
@@ -28,26 +25,26 @@ def cube_structure_beads():
 
 def matrix_to_quat(R):
     """Unit quaternions (w, x, y, z) of rotation matrices (P, 3, 3) (Shepperd's method: the largest of
-    w, x, y, z is taken from the diagonal, the others from off-diagonal sums)."""
+    4w^2, 4x^2, 4y^2, 4z^2 is read from the diagonal, the other components from off-diagonal sums)."""
     R = np.asarray(R, dtype=np.float64)
-    tr = R[:, 0, 0] + R[:, 1, 1] + R[:, 2, 2]
-    cand = np.stack([tr, R[:, 0, 0], R[:, 1, 1], R[:, 2, 2]], axis=1)
-    k = np.argmax(cand, axis=1)
+    m00, m11, m22 = R[:, 0, 0], R[:, 1, 1], R[:, 2, 2]
+    tr = m00 + m11 + m22
+    d10, d01 = R[:, 1, 0], R[:, 0, 1]
+    d20, d02 = R[:, 2, 0], R[:, 0, 2]
+    d21, d12 = R[:, 2, 1], R[:, 1, 2]
+    k = np.argmax(np.stack([tr, m00, m11, m22], axis=1), axis=1)
     q = np.empty((len(R), 4))
-    for idx in range(len(R)):
-        m = R[idx]
-        if k[idx] == 0:
-            s = 2.0 * math.sqrt(max(1.0 + tr[idx], 0.0))
-            q[idx] = (0.25 * s, (m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s)
-        elif k[idx] == 1:
-            s = 2.0 * math.sqrt(max(1.0 + m[0, 0] - m[1, 1] - m[2, 2], 0.0))
-            q[idx] = ((m[2, 1] - m[1, 2]) / s, 0.25 * s, (m[0, 1] + m[1, 0]) / s, (m[0, 2] + m[2, 0]) / s)
-        elif k[idx] == 2:
-            s = 2.0 * math.sqrt(max(1.0 + m[1, 1] - m[0, 0] - m[2, 2], 0.0))
-            q[idx] = ((m[0, 2] - m[2, 0]) / s, (m[0, 1] + m[1, 0]) / s, 0.25 * s, (m[1, 2] + m[2, 1]) / s)
-        else:
-            s = 2.0 * math.sqrt(max(1.0 + m[2, 2] - m[0, 0] - m[1, 1], 0.0))
-            q[idx] = ((m[1, 0] - m[0, 1]) / s, (m[0, 2] + m[2, 0]) / s, (m[1, 2] + m[2, 1]) / s, 0.25 * s)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s0 = 2.0 * np.sqrt(np.maximum(1.0 + tr, 0.0))
+        s1 = 2.0 * np.sqrt(np.maximum(1.0 + m00 - m11 - m22, 0.0))
+        s2 = 2.0 * np.sqrt(np.maximum(1.0 + m11 - m00 - m22, 0.0))
+        s3 = 2.0 * np.sqrt(np.maximum(1.0 + m22 - m00 - m11, 0.0))
+        cand = [np.stack([0.25 * s0, (d21 - d12) / s0, (d02 - d20) / s0, (d10 - d01) / s0], 1),
+                np.stack([(d21 - d12) / s1, 0.25 * s1, (d01 + d10) / s1, (d02 + d20) / s1], 1),
+                np.stack([(d02 - d20) / s2, (d01 + d10) / s2, 0.25 * s2, (d12 + d21) / s2], 1),
+                np.stack([(d10 - d01) / s3, (d02 + d20) / s3, (d12 + d21) / s3, 0.25 * s3], 1)]
+    for c in range(4):
+        q[k == c] = cand[c][k == c]
     return q / np.linalg.norm(q, axis=1, keepdims=True)
 
 
