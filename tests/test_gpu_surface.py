@@ -122,14 +122,14 @@ def test_fd_compat_stencil_matches_oracle(gpu):
     assert dF < noise + 1e-4 * np.abs(F64).max() and dT < noise + 1e-4 * np.abs(T64).max()
 
 
-def _nvt(gpu, net, component=False, fd=False):
+def _nvt(gpu, net, component=False, fd=False, hyp=HYP):
     from nepx.NVT import NVT
     sim = NVT()
     sim.component_path = component
     sim.fd_compat = fd
     sim.setDevice(0)
     sim.initializeSystem(os.path.join(GOLD, "th_init_N200.gsd"))
-    sim.setNepDescriptors([10, 8, 4, 4.5])
+    sim.setNepDescriptors([*hyp, 4.5])
     sim.setEvaluator(net=net)
     sim.setShape("th_v1")
     sim.setNeighborList()
@@ -138,11 +138,14 @@ def _nvt(gpu, net, component=False, fd=False):
     return sim
 
 
-def test_sim_fast_path_equals_method_by_method(gpu):
-    net = _net(gpu)
-    a = _nvt(gpu, net)
+@pytest.mark.parametrize("hyp", [HYP, (10, 10, 10)])
+def test_sim_fast_path_equals_method_by_method(gpu, hyp):
+    """(10, 10, 10): the reference's documented run (sim_input.py:10) with a D = 121 network."""
+    import nepx
+    net = _net(gpu, D=nepx.n_descriptor(*hyp))
+    a = _nvt(gpu, net, hyp=hyp)
     a.run(20)
-    b = _nvt(gpu, net, component=True)
+    b = _nvt(gpu, net, component=True, hyp=hyp)
     b.run(20)
     for prop in ("central_pos", "orientations", "velocities", "angmom"):
         u, v = getattr(a, prop).cpu().numpy(), getattr(b, prop).cpu().numpy()
