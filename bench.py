@@ -68,24 +68,28 @@ def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
     from nepx.configs import make_system
     small = make_system(shape_name, 600, 0.45, seed=123)
     pairs = O.brute_pairs(small.x, small.L, sh["cutoff"])
+    from threadpoolctl import threadpool_limits
     rng = np.random.default_rng(0)
-    t0 = time.perf_counter()
     done = 0
     chunk = 1000
     order = rng.permutation(len(pairs))
-    while time.perf_counter() - t0 < target_s and done < len(pairs):
-        pc = pairs[order[done:done + chunk]]
-        clouds, t = O.pair_clouds(small.x, small.q, pc, sh["sites"], small.L)
-        O.fd_pair_forces(clouds, len(sh["sites"]), t, sh["nep_cutoff"], hyp, layers, "tanh", 0.0, 1.0, 1e-5, 1e-5)
-        done += len(pc)
-    t_pairs = time.perf_counter() - t0
+    with threadpool_limits(limits=1):          # the BLAS calls (network layers) on one thread too
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < target_s and done < len(pairs):
+            pc = pairs[order[done:done + chunk]]
+            clouds, t = O.pair_clouds(small.x, small.q, pc, sh["sites"], small.L)
+            O.fd_pair_forces(clouds, len(sh["sites"]), t, sh["nep_cutoff"], hyp, layers, "tanh", 0.0, 1.0,
+                             1e-5, 1e-5)
+            done += len(pc)
+        t_pairs = time.perf_counter() - t0
+        # integrator per particle (vectorised FP64 numpy)
+        th = dict(xi=0.0, eta=0.0, xi_r=0.0, eta_r=0.0, theta=1.0, e_r=1.0)
+        st = dict(x=spec.x, v=spec.v, q=spec.q, p=spec.p, m=spec.mass)
+        t1 = time.perf_counter()
+        O.nvt_step(st, np.zeros_like(spec.x), np.zeros_like(spec.x), spec.dt, spec.kT, spec.tau, spec.L,
+                   spec.moi, th)
+        per_particle = (time.perf_counter() - t1) / spec.n
     per_pair = t_pairs / done
-    # integrator per particle (vectorised FP64 numpy)
-    th = dict(xi=0.0, eta=0.0, xi_r=0.0, eta_r=0.0, theta=1.0, e_r=1.0)
-    st = dict(x=spec.x, v=spec.v, q=spec.q, p=spec.p, m=spec.mass)
-    t1 = time.perf_counter()
-    O.nvt_step(st, np.zeros_like(spec.x), np.zeros_like(spec.x), spec.dt, spec.kT, spec.tau, spec.L, spec.moi, th)
-    per_particle = (time.perf_counter() - t1) / spec.n
     # pairs per particle of the full workload (half list) at this density
     p_per_particle = len(pairs) / small.n
     step_time = spec.n * (p_per_particle * per_pair + per_particle)
