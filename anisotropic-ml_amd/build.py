@@ -15,7 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "nepx", "libnepx.so")
-SOURCES = ["nepx_api.hip", "nepx_pair.hip", "nepx_nlist.hip", "nepx_integrate.hip", "nepx_bead.hip"]
+SOURCES = ["nepx_api.hip", "nepx_pair.hip", "nepx_quad.hip", "nepx_nlist.hip", "nepx_integrate.hip",
+           "nepx_bead.hip"]
+# Per-source additions: the four-lane fused kernel keeps its matrix-core
+# accumulators in VGPRs, where its VALU code reads them.
+SOURCE_FLAGS = {"nepx_quad.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -27,7 +31,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-munsafe-fp-at
 
 def _compile(src):
     obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + SOURCE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

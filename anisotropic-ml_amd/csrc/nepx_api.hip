@@ -5,7 +5,7 @@
 
 namespace nepx {
 
-static std::atomic<int32_t> g_tune[4] = {{0}, {0}, {0}, {0}};
+static std::atomic<int32_t> g_tune[5] = {{0}, {0}, {0}, {0}, {0}};
 static std::atomic<int32_t> g_cus[64] = {};
 
 nepx_tuning tuning() {
@@ -14,6 +14,7 @@ nepx_tuning tuning() {
     t.pair_blocks_per_cu = g_tune[1].load() ? g_tune[1].load() : 16;
     t.gather_lanes = g_tune[2].load() ? g_tune[2].load() : 16;
     t.net_blocks_per_cu = g_tune[3].load() ? g_tune[3].load() : 2;
+    t.pair_lanes = g_tune[4].load() ? g_tune[4].load() : 4;
     return t;
 }
 
@@ -66,10 +67,13 @@ extern "C" int nepx_set_tuning(const nepx_tuning* t) {
                  "gather_lanes %d not in {4, 8, 16, 32, 64}", gl);
     NEPX_REQUIRE(t->net_blocks_per_cu >= 0 && t->net_blocks_per_cu <= 16, NEPX_EINVAL,
                  "net_blocks_per_cu %d not in 0..16", t->net_blocks_per_cu);
+    NEPX_REQUIRE(t->pair_lanes == 0 || t->pair_lanes == 1 || t->pair_lanes == 4, NEPX_EINVAL,
+                 "pair_lanes %d not in {1, 4}", t->pair_lanes);
     nepx::g_tune[0].store(t->fused_blocks_per_cu);
     nepx::g_tune[1].store(t->pair_blocks_per_cu);
     nepx::g_tune[2].store(gl);
     nepx::g_tune[3].store(t->net_blocks_per_cu);
+    nepx::g_tune[4].store(t->pair_lanes);
     return NEPX_OK;
 }
 

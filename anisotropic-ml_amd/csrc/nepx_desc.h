@@ -44,6 +44,10 @@ struct Harm {
     float C[LMAX + 1], S[LMAX + 1];
 
     __device__ __forceinline__ float q(int l, int m) const { return Q[T::qi(l, m)]; }
+    // Z_l0 = Q_l^0; Z_lm = Q_l^m C_m (part 0) or Q_l^m S_m (part 1) for m >= 1.
+    __device__ __forceinline__ float z(int l, int m, int part) const {
+        return m == 0 ? q(l, 0) : q(l, m) * (part ? S[m] : C[m]);
+    }
 
     __device__ __forceinline__ void eval(float x, float y, float z) {
         Q[0] = 1.f;
@@ -175,10 +179,11 @@ struct SiteGeo {
     float r, ir, rho;  // |r_a|, 1/|r_a|, |r_a|/r_nep
 };
 
-__device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3& R, int k, float ox, float oy,
-                                            float oz, float inv_rn) {
+// Body-frame site b, body rotation R, body centre minus cloud centroid o.
+__device__ __forceinline__ SiteGeo site_geo_b(const Mat3& R, float bx, float by, float bz, float ox, float oy,
+                                              float oz, float inv_rn) {
     SiteGeo s;
-    mat_vec(R, m.sites[3 * k], m.sites[3 * k + 1], m.sites[3 * k + 2], s.ax, s.ay, s.az);
+    mat_vec(R, bx, by, bz, s.ax, s.ay, s.az);
     const float rx = s.ax + ox, ry = s.ay + oy, rz = s.az + oz;
     const float r2 = rx * rx + ry * ry + rz * rz;
     s.ir = __builtin_amdgcn_rsqf(r2);  // hardware 1/sqrt (about 1 ulp)
@@ -188,6 +193,65 @@ __device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3
     s.uz = rz * s.ir;
     s.rho = s.r * inv_rn;
     return s;
+}
+
+__device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3& R, int k, float ox, float oy,
+                                            float oz, float inv_rn) {
+    return site_geo_b(R, m.sites[3 * k], m.sites[3 * k + 1], m.sites[3 * k + 2], ox, oy, oz, inv_rn);
+}
+
+// Radial functions at rho: fn(n, f_n) for n = 0..NMAX with
+// f_n = (T_n(x) + 1) w fc/2,  x = 2(rho-1)^2 - 1,  fc = (1 + cos(pi rho))/2.
+template <int NMAX, class Fn>
+__device__ __forceinline__ void radial_values(float rho, float w, Fn&& fn) {
+    const float x = 2.f * (rho - 1.f) * (rho - 1.f) - 1.f;
+    const float hf = 0.25f * (1.f + cospif(rho)) * w;  // w fc / 2
+    float t0 = 1.f, t1 = x;
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+        float tn;
+        if (n == 0) tn = 1.f;
+        else if (n == 1) tn = x;
+        else {
+            tn = 2.f * x * t1 - t0;
+            t0 = t1;
+            t1 = tn;
+        }
+        fn(n, tn * hf + hf);
+    }
+}
+
+// Radial functions and their rho-derivatives: fn(n, f_n, f_n') for n = 0..NMAX, with
+// f_n' = T_n'(x) x' fc/2 + (T_n(x) + 1) fc'/2 (unit weight).
+template <int NMAX, class Fn>
+__device__ __forceinline__ void radial_derivs(float rho, Fn&& fn) {
+    const float xm = rho - 1.f;
+    const float x = 2.f * xm * xm - 1.f;
+    float sp, cp;
+    sincospif(rho, &sp, &cp);
+    const float hf = 0.25f * (1.f + cp);                // fc / 2
+    const float hdx = 2.f * xm * (0.5f * (1.f + cp));    // (dx/drho) fc / 2
+    const float hdf = -0.25f * (float)M_PI * sp;         // fc' / 2
+    float t0 = 1.f, t1 = x, d0 = 0.f, d1 = 1.f;
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+        float tn, dn;
+        if (n == 0) {
+            tn = 1.f;
+            dn = 0.f;
+        } else if (n == 1) {
+            tn = x;
+            dn = 1.f;
+        } else {
+            tn = 2.f * x * t1 - t0;
+            dn = 2.f * t1 + 2.f * x * d1 - d0;
+            t0 = t1;
+            t1 = tn;
+            d0 = d1;
+            d1 = dn;
+        }
+        fn(n, tn * hf + hf, dn * hdx + (tn * hdf + hdf));
+    }
 }
 
 // Site visitors.  visit(site, body_done) calls site(s, w) for every site
@@ -266,34 +330,20 @@ __device__ __forceinline__ void accumulate_sites(const Sites& sites, float (&gra
         for (int k = 0; k < LR::N; ++k) A[n][k] = 0.f;
     sites.visit(
         [&](const SiteGeo& s, float w) {
-            const float x = 2.f * (s.rho - 1.f) * (s.rho - 1.f) - 1.f;
-            const float hf = 0.25f * (1.f + cospif(s.rho)) * w;  // w fc / 2
             float fa[Cfg::NANG + 1];
-            float t0 = 1.f, t1 = x;
-#pragma unroll
-            for (int n = 0; n <= (RAD ? Cfg::NRAD : Cfg::NANG); ++n) {
-                float tn;
-                if (n == 0) tn = 1.f;
-                else if (n == 1) tn = x;
-                else {
-                    tn = 2.f * x * t1 - t0;
-                    t0 = t1;
-                    t1 = tn;
-                }
-                const float f = tn * hf + hf;
+            radial_values<RAD ? Cfg::NRAD : Cfg::NANG>(s.rho, w, [&](int n, float f) {
                 if constexpr (RAD) grad[n] += f;
                 if (n <= Cfg::NANG) fa[n] = f;
-            }
+            });
             Harm<LR::L1> h;
             h.eval(s.ux, s.uy, s.uz);
 #pragma unroll
             for (int l = LR::L0; l <= LR::L1; ++l) {
 #pragma unroll
                 for (int mm = 0; mm <= l; ++mm) {
-                    const float ql = h.q(l, mm);
 #pragma unroll
                     for (int part = 0; part < (mm ? 2 : 1); ++part) {
-                        const float z = mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]);
+                        const float z = h.z(l, mm, part);
                         const int o = lm_slot(l, mm, part) - LR::OFF;
 #pragma unroll
                         for (int n = 0; n <= Cfg::NANG; ++n) A[n][o] += fa[n] * z;
@@ -331,6 +381,67 @@ __device__ __forceinline__ void desc_finish(const float (&A)[Cfg::NANG + 1][LR::
     }
 }
 
+// Angular part of the adjoint at one site.  cd(o, c, d) supplies, for slot o of
+// range LR, c_lm = sum_n B_nlm f_n(rho_a) and d_lm = sum_n B_nlm f_n'(rho_a).
+// Adds sum_lm d_lm Z_lm to dEdrho and returns
+//   (gx, gy, gz) = grad sum_lm c_lm Z_lm  (Z_lm extended to homogeneous polynomials of degree l)
+//   proj         = sum_lm l c_lm Z_lm(u_a),
+// using  d/dx Q_l^m = x Q_{l-1}^{m+1},  d/dy Q_l^m = y Q_{l-1}^{m+1},  d/dz Q_l^m = (l+m) Q_{l-1}^m,
+//        d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}),
+// with the sums over l taken first for each m, so each (l, m) costs a few FMAs.
+template <class LR, class CD>
+__device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, CD&& cd,
+                                                  float& dEdrho, float& gx, float& gy, float& gz, float& proj) {
+    float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f;
+    proj = 0.f;
+#pragma unroll
+    for (int mm = 0; mm <= LR::L1; ++mm) {
+        float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
+#pragma unroll
+        for (int l = (mm > LR::L0 ? mm : LR::L0); l <= LR::L1; ++l) {
+            const float ql = h.q(l, mm);
+            const float qa = (mm + 1 <= l - 1) ? h.q(l - 1, mm + 1) : 0.f;
+            const float qc = (mm <= l - 1) ? (float)(l + mm) * h.q(l - 1, mm) : 0.f;
+            float pl = 0.f;
+#pragma unroll
+            for (int part = 0; part < (mm ? 2 : 1); ++part) {
+                const int o = lm_slot(l, mm, part) - LR::OFF;
+                const float z = mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]);
+                float c, d;
+                cd(o, c, d);
+                dEdrho += d * z;
+                pl += c * z;
+                if (mm + 1 <= l - 1) U[part] += qa * c;
+                if (mm <= l - 1) V[part] += qc * c;
+                if (mm >= 1) W[part] += ql * c;
+            }
+            proj += (float)l * pl;
+        }
+        if (mm == 0) {
+            E1 += U[0];
+            Gz += V[0];
+        } else {
+            E1 += h.C[mm] * U[0] + h.S[mm] * U[1];
+            Gz += h.C[mm] * V[0] + h.S[mm] * V[1];
+            const float fm = (float)mm;
+            X1 += fm * (h.C[mm - 1] * W[0] + h.S[mm - 1] * W[1]);
+            Y1 += fm * (h.C[mm - 1] * W[1] - h.S[mm - 1] * W[0]);
+        }
+    }
+    gx = ux * E1 + X1;
+    gy = uy * E1 + Y1;
+    gz = Gz;
+}
+
+// dE/dr_a = dE/drho_a u_a / r_nep + (grad - proj u_a) / |r_a|.
+__device__ __forceinline__ void site_dEdr(const SiteGeo& s, float dEdrho, float inv_rn, float gx, float gy, float gz,
+                                          float proj, float& ox, float& oy, float& oz) {
+    const float rad = dEdrho * inv_rn;
+    ox = rad * s.ux + s.ir * (gx - proj * s.ux);
+    oy = rad * s.uy + s.ir * (gy - proj * s.uy);
+    oz = rad * s.uz + s.ir * (gz - proj * s.uz);
+}
+
 // Per-pair result of the adjoint.
 struct PairOut {
     float fx, fy, fz;     // F on i (F on j is -F)
@@ -351,11 +462,7 @@ struct AdjAcc {
 //   dE/drho_a = sum_n dE/dg_rad[n] f_n'(rho_a) + sum_lm d_lm Z_lm(u_a)
 //   dE/dr_a   = dE/drho_a u_a / r_nep
 //             + (1/|r_a|) [ grad sum_lm c_lm Z_lm (u_a) - u_a sum_lm l c_lm Z_lm(u_a) ]
-// where Z_lm is extended to the homogeneous polynomial of degree l, whose
-// gradient follows from
-//   d/dx Q_l^m = x Q_{l-1}^{m+1},  d/dy Q_l^m = y Q_{l-1}^{m+1},  d/dz Q_l^m = (l+m) Q_{l-1}^m,
-//   d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}).
-// The sums over l are taken first for each m, so each (l, m) costs a few FMAs.
+// where Z_lm is extended to the homogeneous polynomial of degree l (angular_site_grad).
 // Finally dE/dp_a = dE/dr_a - mean_b dE/dr_b through the centroid (adj_finish).
 // One pass handles the l of range LR; the radial terms and the arms go with RAD.
 template <class Cfg, class LR, bool RAD>
@@ -382,89 +489,31 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
     float ax_s = 0.f, ay_s = 0.f, az_s = 0.f;
     PairSites<Cfg>{m, p, inv_rn}.visit(
         [&](const SiteGeo& s, float) {
-            const float xm = s.rho - 1.f;
-            const float x = 2.f * xm * xm - 1.f;
-            float sp, cp;
-            sincospif(s.rho, &sp, &cp);
-            const float hf = 0.25f * (1.f + cp);                      // fc / 2
-            const float hdx = 2.f * xm * (0.5f * (1.f + cp));          // (dx/drho) fc / 2
-            const float hdf = -0.25f * (float)M_PI * sp;               // fc' / 2
-            // f_n = (T_n + 1) fc/2,  f_n' = T_n' x' fc/2 + (T_n + 1) fc'/2
             float fa[Cfg::NANG + 1], da[Cfg::NANG + 1];
             float dEdrho = 0.f;
-            {
-                float t0 = 1.f, t1 = x, d0 = 0.f, d1 = 1.f;
-#pragma unroll
-                for (int n = 0; n <= (RAD ? Cfg::NRAD : Cfg::NANG); ++n) {
-                    float tn, dn;
-                    if (n == 0) {
-                        tn = 1.f;
-                        dn = 0.f;
-                    } else if (n == 1) {
-                        tn = x;
-                        dn = 1.f;
-                    } else {
-                        tn = 2.f * x * t1 - t0;
-                        dn = 2.f * t1 + 2.f * x * d1 - d0;
-                        t0 = t1;
-                        t1 = tn;
-                        d0 = d1;
-                        d1 = dn;
-                    }
-                    const float df = dn * hdx + (tn * hdf + hdf);
-                    if constexpr (RAD) dEdrho += dG[n] * df;
-                    if (n <= Cfg::NANG) {
-                        fa[n] = tn * hf + hf;
-                        da[n] = df;
-                    }
+            radial_derivs<RAD ? Cfg::NRAD : Cfg::NANG>(s.rho, [&](int n, float f, float df) {
+                if constexpr (RAD) dEdrho += dG[n] * df;
+                if (n <= Cfg::NANG) {
+                    fa[n] = f;
+                    da[n] = df;
                 }
-            }
+            });
             Harm<LR::L1> h;
             h.eval(s.ux, s.uy, s.uz);
-            float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f, proj = 0.f;
+            float gx, gy, gz, proj;
+            angular_site_grad<LR>(h, s.ux, s.uy,
+                                  [&](int o, float& c, float& d) {
+                                      c = 0.f;
+                                      d = 0.f;
 #pragma unroll
-            for (int mm = 0; mm <= LR::L1; ++mm) {
-                float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
-#pragma unroll
-                for (int l = (mm > LR::L0 ? mm : LR::L0); l <= LR::L1; ++l) {
-                    const float ql = h.q(l, mm);
-                    const float qa = (mm + 1 <= l - 1) ? h.q(l - 1, mm + 1) : 0.f;
-                    const float qc = (mm <= l - 1) ? (float)(l + mm) * h.q(l - 1, mm) : 0.f;
-                    float pl = 0.f;
-#pragma unroll
-                    for (int part = 0; part < (mm ? 2 : 1); ++part) {
-                        const int o = lm_slot(l, mm, part) - LR::OFF;
-                        const float z = mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]);
-                        float c = 0.f, d = 0.f;
-#pragma unroll
-                        for (int n = 0; n <= Cfg::NANG; ++n) {
-                            c += A[n][o] * fa[n];
-                            d += A[n][o] * da[n];
-                        }
-                        dEdrho += d * z;
-                        pl += c * z;
-                        if (mm + 1 <= l - 1) U[part] += qa * c;
-                        if (mm <= l - 1) V[part] += qc * c;
-                        if (mm >= 1) W[part] += ql * c;
-                    }
-                    proj += (float)l * pl;
-                }
-                if (mm == 0) {
-                    E1 += U[0];
-                    Gz += V[0];
-                } else {
-                    E1 += h.C[mm] * U[0] + h.S[mm] * U[1];
-                    Gz += h.C[mm] * V[0] + h.S[mm] * V[1];
-                    const float fm = (float)mm;
-                    X1 += fm * (h.C[mm - 1] * W[0] + h.S[mm - 1] * W[1]);
-                    Y1 += fm * (h.C[mm - 1] * W[1] - h.S[mm - 1] * W[0]);
-                }
-            }
-            const float gx = s.ux * E1 + X1, gy = s.uy * E1 + Y1, gz = Gz;
-            const float rad = dEdrho * inv_rn;
-            const float ga_x = rad * s.ux + s.ir * (gx - proj * s.ux);
-            const float ga_y = rad * s.uy + s.ir * (gy - proj * s.uy);
-            const float ga_z = rad * s.uz + s.ir * (gz - proj * s.uz);
+                                      for (int n = 0; n <= Cfg::NANG; ++n) {
+                                          c += A[n][o] * fa[n];
+                                          d += A[n][o] * da[n];
+                                      }
+                                  },
+                                  dEdrho, gx, gy, gz, proj);
+            float ga_x, ga_y, ga_z;
+            site_dEdr(s, dEdrho, inv_rn, gx, gy, gz, proj, ga_x, ga_y, ga_z);
             gx_s += ga_x;
             gy_s += ga_y;
             gz_s += ga_z;

@@ -1,72 +1,10 @@
 // Pair kernels: descriptor forward, MFMA energy network, adjoint, and the
 // fused step kernel.  See nepx_desc.h and nepx_ffnn.h for the math and layouts.
-#include "nepx_desc.h"
-#include "nepx_ffnn.h"
+#include "nepx_pairk.h"
 
 #include <string.h>
 
 namespace nepx {
-
-constexpr int kBlock = 256;
-
-// Occupancy target of the register-heavy adjoint/fused kernels (waves per SIMD).
-#ifndef NEPX_ADJ_WAVES
-#define NEPX_ADJ_WAVES 1
-#endif
-#define NEPX_ADJ_ATTR __attribute__((amdgpu_waves_per_eu(NEPX_ADJ_WAVES, NEPX_ADJ_WAVES)))
-
-__device__ __forceinline__ int n_pairs(const int32_t* np_dev, int64_t maxp) {
-    const int p = *np_dev;
-    return p < maxp ? p : (int)maxp;
-}
-
-__device__ __forceinline__ void load_pair(const float4* __restrict__ qf, const int32_t* __restrict__ pi,
-                                          const int32_t* __restrict__ pj, const float4* __restrict__ pt,
-                                          int idx, float4& qi, float4& qj, float4& t) {
-    const int i = pi[idx];
-    const int j = pj[idx];
-    t = pt[idx];
-    qi = qf[i];
-    qj = qf[j];
-}
-
-template <class Cfg>
-__device__ __forceinline__ void store_row(float* __restrict__ dst, const float (&v)[Cfg::D]) {
-    if constexpr (Cfg::D % 4 == 0) {
-        float4* d4 = reinterpret_cast<float4*>(dst);
-#pragma unroll
-        for (int k = 0; k < Cfg::D / 4; ++k) d4[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
-    } else {
-#pragma unroll
-        for (int k = 0; k < Cfg::D; ++k) dst[k] = v[k];
-    }
-}
-
-template <class Cfg>
-__device__ __forceinline__ void load_row(const float* __restrict__ src, float (&v)[Cfg::D]) {
-    if constexpr (Cfg::D % 4 == 0) {
-        const float4* s4 = reinterpret_cast<const float4*>(src);
-#pragma unroll
-        for (int k = 0; k < Cfg::D / 4; ++k) {
-            const float4 x = s4[k];
-            v[4 * k] = x.x;
-            v[4 * k + 1] = x.y;
-            v[4 * k + 2] = x.z;
-            v[4 * k + 3] = x.w;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < Cfg::D; ++k) v[k] = src[k];
-    }
-}
-
-__device__ __forceinline__ void store_out(float* __restrict__ out, int idx, const PairOut& o, float E,
-                                          float clip) {
-    float4* o4 = reinterpret_cast<float4*>(out) + 3 * (size_t)idx;
-    o4[0] = make_float4(clipf(o.fx, clip), clipf(o.fy, clip), clipf(o.fz, clip), E);
-    o4[1] = make_float4(clipf(o.tix, clip), clipf(o.tiy, clip), clipf(o.tiz, clip), 0.f);
-    o4[2] = make_float4(clipf(o.tjx, clip), clipf(o.tjy, clip), clipf(o.tjz, clip), 0.f);
-}
 
 // Descriptor row of one pair (or cloud), one pass over the sites per l range.
 template <class Cfg, class Sites, class Row>
@@ -144,80 +82,6 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_mode
         AdjAcc acc;
         adjoint_all<Cfg>(m, g, dG, acc);
         store_out(out, idx, adj_finish<Cfg>(acc), E ? E[idx] : 0.f, clip);
-    }
-}
-
-// Input chunks of one pair for the MFMA tile: chunk c holds inputs
-// 16c + 4*(lane/16) + 0..3 (zero past D).
-template <int MC>
-__device__ __forceinline__ void gather_chunks(const float* __restrict__ row, bool valid, int D, v4f (&gin)[MC]) {
-    const int g = (threadIdx.x & 63) >> 4;
-#pragma unroll
-    for (int c = 0; c < MC; ++c) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int k = 16 * c + 4 * g + r;
-            gin[c][r] = (valid && k < D) ? row[k] : 0.f;
-        }
-    }
-}
-
-template <int MC>
-__device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool valid, int D, const v4f (&v)[MC]) {
-    const int g = (threadIdx.x & 63) >> 4;
-#pragma unroll
-    for (int c = 0; c < MC; ++c) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int k = 16 * c + 4 * g + r;
-            if (valid && k < D) row[k] = v[c][r];
-        }
-    }
-}
-
-// Networks whose packed operands fit this budget are staged in LDS once per block.
-constexpr int kWeightLdsBytes = 64 * 1024;
-template <class N, bool HALF>
-constexpr int operand_floats() {
-    return HALF ? N::layout_h().total() / 2 : N::layout().total();
-}
-template <class N, bool HALF>
-constexpr bool weights_in_lds() {
-    return operand_floats<N, HALF>() * 4 <= kWeightLdsBytes;
-}
-template <class N, bool HALF>
-constexpr int weight_lds_floats() {
-    return weights_in_lds<N, HALF>() ? operand_floats<N, HALF>() : 4;
-}
-
-constexpr int kNetTiles = 2;  // 16-pair tiles sharing each A-operand read
-
-// Network over NT tiles of 16 pairs with the operands chosen at compile time.
-template <class N, int NT, bool HALF>
-struct NetRun {
-    static constexpr int IC = HALF ? N::MDH : N::MD;  // input chunks
-    __device__ __forceinline__ static void run(const float* W, const _Float16* WH, int act, const v4f (&gin)[NT][IC],
-                                               float scale, v4f (&dg)[NT][N::MD], float (&y)[NT]) {
-        if constexpr (HALF) ffnn_tiles_h<N, NT>(W, WH, act, gin, scale, dg, y);
-        else ffnn_tiles<N, NT>(W, act, gin, scale, dg, y);
-    }
-};
-
-// Stage the operands of this precision into LDS when they fit; returns the pointers to use.
-template <class N, bool HALF>
-__device__ __forceinline__ void stage_net(float* wl, const float* Wg, const void* Whg, const float*& W,
-                                          const _Float16*& WH) {
-    W = Wg;
-    WH = reinterpret_cast<const _Float16*>(Whg);
-    if constexpr (weights_in_lds<N, HALF>()) {
-        if constexpr (HALF) {
-            stage_copy(wl, reinterpret_cast<const float*>(Whg), operand_floats<N, HALF>());
-            WH = reinterpret_cast<const _Float16*>(wl);
-        } else {
-            stage_copy(wl, Wg, operand_floats<N, HALF>());
-            W = wl;
-        }
-        __syncthreads();
     }
 }
 
@@ -423,19 +287,6 @@ __global__ __launch_bounds__(kBlock) void k_fd_forces(const float* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Dispatch tables.
-
-using CfgCube8 = DescCfg<8, 11, 6, 4>;     // cube, 8 vertices, D = 40
-using CfgTet10 = DescCfg<10, 11, 6, 4>;    // tetrahedron, 4 vertices + 6 mid-edges, D = 40
-using CfgCyl12 = DescCfg<12, 17, 6, 6>;    // twisted cylinder, 12 helix sites, D = 60
-using CfgCubeV2 = DescCfg<6, 10, 8, 4>;    // reference cube_v2 with hypers 10 8 4, D = 47
-using CfgThV1 = DescCfg<4, 10, 8, 4>;      // reference th_v1 with hypers 10 8 4, D = 47
-using CfgCubeV2L10 = DescCfg<6, 10, 10, 10>;  // reference cube_v2 with hypers 10 10 10, D = 121
-using CfgThV1L10 = DescCfg<4, 10, 10, 10>;    // reference th_v1 with hypers 10 10 10, D = 121
-
-using Net40x64 = NetCfg<48, 64, 2>;        // 40 -> 64 -> 64 -> 1
-using Net60x128 = NetCfg<64, 128, 2>;      // 60 -> 128 -> 128 -> 1
-using Net47x150 = NetCfg<48, 160, 4>;      // reference TheEnergyNet(150, 3, 47): 47 -> 150 x4 -> 1
-using Net121x150 = NetCfg<128, 160, 4>;    // reference TheEnergyNet(150, 3, 121): 121 -> 150 x4 -> 1
 
 static int grid_for_cap(int64_t units, int per_block, int64_t cap) {
     int64_t g = (units + per_block - 1) / per_block;
@@ -700,9 +551,17 @@ extern "C" int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_ne
 }
 
 template <class Cfg, class N>
-static int launch_fused(const nepx_pair_model& m, const nepx_net* net, bool half, int grid, float scale,
+static int launch_fused(const nepx_pair_model& m, const nepx_net* net, bool half, float scale,
                         const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
                         const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+    const int64_t cap = (int64_t)device_cus() * tuning().fused_blocks_per_cu;
+    // four lanes per pair (site sums on MFMA) where the shape allows it, unless tuned off
+    if (tuning().pair_lanes == 4) {
+        const int rc = launch_fused_quad<Cfg, N>(m, net, half, (int)cap, scale, qf, pair_i, pair_j, pair_t,
+                                                 np_dev, maxp, clip, out, stream);
+        if (rc != NEPX_EUNSUP) return rc;
+    }
+    const int grid = grid_for_cap(maxp, kBlock, cap);
     const bool tanh_act = net->act == NEPX_ACT_TANH;
 #define NEPX_LAUNCH_FUSED(H, A)                                                                                  \
     hipLaunchKernelGGL((k_fused<Cfg, N, H, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), m, net->w_packed, \
@@ -730,13 +589,12 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && out, NEPX_EINVAL, "null buffer");
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-    const int grid = grid_for_cap(maxp, kBlock, (int64_t)device_cus() * tuning().fused_blocks_per_cu);
     const bool half = net->precision == NEPX_NET_F16X3;
     NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
 #define X(ns, a, b, c, T, d, w, h, NT)                                                               \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&        \
         net->width == w && net->n_hidden == h)                                                       \
-        return launch_fused<T, NT>(*m, net, half, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip, \
+        return launch_fused<T, NT>(*m, net, half, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip,       \
                                    out, stream);
     NEPX_FUSED_CASES(X)
 #undef X

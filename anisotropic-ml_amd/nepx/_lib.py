@@ -53,7 +53,7 @@ class NList(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("fused_blocks_per_cu", C.c_int32), ("pair_blocks_per_cu", C.c_int32),
-                ("gather_lanes", C.c_int32), ("net_blocks_per_cu", C.c_int32)]
+                ("gather_lanes", C.c_int32), ("net_blocks_per_cu", C.c_int32), ("pair_lanes", C.c_int32)]
 
 
 # numpy mirrors of the per-system records

@@ -1,11 +1,15 @@
 """Autotuner for the step's launch shapes and the Verlet-list skin.
 
-What is tuned (none of it changes which pairs interact or any per-pair value):
+What is tuned (none of it changes which pairs interact; only pair_lanes changes per-pair values,
+at FP32 rounding level):
   skin                 Verlet skin: list length (filter cost every step) against rebuild frequency
   gather_lanes         lanes per particle in the deterministic force gather (4 .. 64)
   fused_blocks_per_cu  grid of the fused descriptor/network/adjoint kernel (persistent blocks per CU)
   pair_blocks_per_cu   grid cap of the split descriptor and adjoint kernels
   net_blocks_per_cu    grid cap of the split network kernel
+  pair_lanes           lanes per pair in the fused kernel: 4 puts the per-pair site sums on the
+                       matrix cores (default where the shape allows), 1 keeps one lane per pair;
+                       the two differ in FP32 rounding only, so it is not searched by default
 
 Each candidate is measured on the workload itself, on a scratch copy of the
 systems: the launch parameters are set process-wide (nepx_set_tuning), the
@@ -33,7 +37,7 @@ import torch
 
 from . import _lib
 
-KNOBS = ("fused_blocks_per_cu", "pair_blocks_per_cu", "gather_lanes", "net_blocks_per_cu")
+KNOBS = ("fused_blocks_per_cu", "pair_blocks_per_cu", "gather_lanes", "net_blocks_per_cu", "pair_lanes")
 
 
 @dataclass
@@ -43,6 +47,7 @@ class Tuning:
     pair_blocks_per_cu: int = 16
     gather_lanes: int = 16
     net_blocks_per_cu: int = 2
+    pair_lanes: int = 4
 
     def launch(self):
         t = _lib.Tuning()

@@ -137,6 +137,8 @@ typedef struct nepx_tuning {
     int32_t pair_blocks_per_cu;    /* grid cap of the descriptor / adjoint kernels (1..64; default 16)  */
     int32_t gather_lanes;          /* lanes per particle in the force gather: 4..64, power of 2 (16)   */
     int32_t net_blocks_per_cu;     /* grid cap of the split network kernel (1..16; default 2)          */
+    int32_t pair_lanes;            /* lanes per pair in the fused kernel: 4 = site sums on MFMA where the
+                                      shape allows (default), 1 = one lane per pair (site sums on VALU) */
 } nepx_tuning;
 int nepx_set_tuning(const nepx_tuning* t);
 int nepx_get_tuning(nepx_tuning* t);

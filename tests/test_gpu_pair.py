@@ -101,10 +101,23 @@ def test_adjoint_matches_oracle(gpu, shape_name, hyp):
     assert max(eF, eTi, eTj) < 1e-4
 
 
+@pytest.fixture(params=[4, 1], ids=["lanes4", "lanes1"])
+def pair_lanes(request):
+    """Fused kernel variant: four lanes per pair (site sums on MFMA) or one lane per pair."""
+    from nepx import tune
+    old = tune.get_launch()
+    tune.set_launch(tune.Tuning(**{**old, "pair_lanes": request.param}))
+    yield request.param
+    tune.set_launch(tune.Tuning(**old))
+
+
 @pytest.mark.parametrize("precision", ["f32", "f16x3"])
 @pytest.mark.parametrize("shape_name,hyp,widths", [("tet10", (11, 6, 4), [64, 64]), ("cube8", (11, 6, 4), [64, 64]),
                                                    ("cyl12", (17, 6, 6), [128, 128])])
-def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision):
+def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision, pair_lanes):
+    """The fused kernel equals descriptor -> network -> adjoint run as separate kernels to FP32
+    rounding (the one-lane kernel sums in the same order; the four-lane kernel sums the sites on
+    matrix cores and over lanes, so it is held to 1e-5), and FP64 end to end."""
     import nepx
     from nepx import ops
     shape, (q, i, j, t), T = _inputs(gpu, shape_name, n_pairs=1000, seed=9)
@@ -116,13 +129,18 @@ def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision):
     ref = ops.descriptor_adjoint(model, T["qf"], T["pi"], T["pj"], T["pt"], dG, E=E, clip=100.0).cpu().numpy()
     fused = ops.pair_fused(model, net, T["qf"], T["pi"], T["pj"], T["pt"], clip=100.0).cpu().numpy()
     e = normwise_rel(fused, ref)
-    print(f"fused vs pipeline {shape_name} {precision}: {e:.3e}")
-    assert e < 1e-6
-    # and against FP64 end to end (pair energy + analytic forces)
+    print(f"fused ({pair_lanes} lanes/pair) vs pipeline {shape_name} {precision}: {e:.3e}")
+    assert e < (1e-6 if pair_lanes == 1 else 1e-5)
+    # and against FP64 end to end (pair energy, analytic forces and both torques)
     clouds = oracle_clouds(q, i, j, t, shape.sites)
     G64 = O.descriptor(clouds, shape.nep_cutoff, *hyp)
     y, g = O.ffnn_forward_backward(G64, net.layers_f64(), "tanh")
     gp = O.descriptor_grad(clouds, g, shape.nep_cutoff, *hyp)
     Ns = shape.n_sites
-    assert normwise_rel(fused[:, 3], y) < 1e-5
-    assert normwise_rel(fused[:, 0:3], -gp[:, :Ns].sum(1)) < 1e-4
+    Ti = np.cross(clouds[:, :Ns], -gp[:, :Ns]).sum(1)
+    Tj = np.cross(clouds[:, Ns:] - t.astype(np.float64)[:, None, :], -gp[:, Ns:]).sum(1)
+    eE, eF = normwise_rel(fused[:, 3], y), normwise_rel(fused[:, 0:3], -gp[:, :Ns].sum(1))
+    eTi, eTj = normwise_rel(fused[:, 4:7], Ti), normwise_rel(fused[:, 8:11], Tj)
+    print(f"  vs FP64: E {eE:.3e} F {eF:.3e} tau_i {eTi:.3e} tau_j {eTj:.3e}")
+    assert eE < 1e-5
+    assert max(eF, eTi, eTj) < 1e-4

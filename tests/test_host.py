@@ -273,10 +273,16 @@ def test_tuning_roundtrip_and_validation():
         bad.gather_lanes = 12
         assert _lib.lib.nepx_set_tuning(ctypes.byref(bad)) == -1
         assert b"gather_lanes" in _lib.lib.nepx_last_error()
+        bad = _lib.Tuning()
+        bad.pair_lanes = 2
+        assert _lib.lib.nepx_set_tuning(ctypes.byref(bad)) == -1
+        assert b"pair_lanes" in _lib.lib.nepx_last_error()
         tune.set_launch(tune.Tuning(gather_lanes=0, fused_blocks_per_cu=0, pair_blocks_per_cu=0,
-                                    net_blocks_per_cu=0))
+                                    net_blocks_per_cu=0, pair_lanes=0))
         assert tune.get_launch() == {"fused_blocks_per_cu": 1, "pair_blocks_per_cu": 16, "gather_lanes": 16,
-                                     "net_blocks_per_cu": 2}
+                                     "net_blocks_per_cu": 2, "pair_lanes": 4}
+        tune.set_launch(tune.Tuning(pair_lanes=1))
+        assert tune.get_launch()["pair_lanes"] == 1
     finally:
         tune.set_launch(tune.Tuning(**old))
 
