@@ -544,7 +544,7 @@ extern "C" int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_ne
 #define X(ns, a, b, c, T, d, w, h, NT)                                                            \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&    \
         net->width == w && net->n_hidden == h)                                                   \
-        return 1;
+        return QuadCfg<T>::OK && tuning().pair_lanes == 4 ? 4 : 1;
     NEPX_FUSED_CASES(X)
 #undef X
     return 0;

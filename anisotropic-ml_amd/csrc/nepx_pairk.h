@@ -156,6 +156,25 @@ using Net60x128 = NetCfg<64, 128, 2>;      // 60 -> 128 -> 128 -> 1
 using Net47x150 = NetCfg<48, 160, 4>;      // reference TheEnergyNet(150, 3, 47): 47 -> 150 x4 -> 1
 using Net121x150 = NetCfg<128, 160, 4>;    // reference TheEnergyNet(150, 3, 121): 121 -> 150 x4 -> 1
 
+// Layout of the four-lanes-per-pair fused kernel (nepx_quad.h).
+template <class Cfg>
+struct QuadCfg {
+    static constexpr int NC = (Cfg::NANG + 1 + 3) / 4;  // 4-row chunks of n
+    static constexpr int LC = (Cfg::NLM + 3) / 4;       // 4-column chunks of lm
+    static constexpr int STEPS = Cfg::ND / 4;           // sites per lane
+    static constexpr int CH = NC + LC;                  // exchanged chunks per site step
+    // LDS floats per pair for the exchange; TS/4 odd puts the 64 scalar writes of
+    // one wave instruction on 64 distinct banks
+    static constexpr int TS = CH * 16 + 4;
+    // forward and adjoint accumulators of up to 2 x 96 registers per lane
+    static constexpr bool OK = Cfg::ND % 4 == 0 && NC * LC <= 24;
+    // Shapes whose lane state fits 256 registers run two waves per SIMD (512-thread
+    // blocks sharing one copy of the network weights), so one wave's LDS and
+    // matrix-core latencies overlap the other's work; larger ones run one.
+    static constexpr int WAVES_PER_SIMD = NC * LC <= 12 ? 2 : 1;
+    static constexpr int BLOCK = 256 * WAVES_PER_SIMD;
+};
+
 // Fused descriptor -> network -> adjoint with four lanes per pair and the site
 // sums on matrix cores (nepx_quad.hip), on at most max_blocks blocks.  Returns
 // NEPX_EUNSUP for shapes it does not cover.

@@ -28,24 +28,6 @@
 
 namespace nepx {
 
-template <class Cfg>
-struct QuadCfg {
-    static constexpr int NC = (Cfg::NANG + 1 + 3) / 4;  // 4-row chunks of n
-    static constexpr int LC = (Cfg::NLM + 3) / 4;       // 4-column chunks of lm
-    static constexpr int STEPS = Cfg::ND / 4;           // sites per lane
-    static constexpr int CH = NC + LC;                  // exchanged chunks per site step
-    // LDS floats per pair for the exchange; TS/4 odd puts the 64 scalar writes of
-    // one wave instruction on 64 distinct banks
-    static constexpr int TS = CH * 16 + 4;
-    // forward and adjoint accumulators of up to 2 x 96 registers per lane
-    static constexpr bool OK = Cfg::ND % 4 == 0 && NC * LC <= 24;
-    // Shapes whose lane state fits 256 registers run two waves per SIMD (512-thread
-    // blocks sharing one copy of the network weights), so one wave's LDS and
-    // matrix-core latencies overlap the other's work; larger ones run one.
-    static constexpr int WAVES_PER_SIMD = NC * LC <= 12 ? 2 : 1;
-    static constexpr int BLOCK = 256 * WAVES_PER_SIMD;
-};
-
 // l and m of accumulator slot lm (inverse of lm_slot).
 __host__ __device__ constexpr int l_of_slot(int lm) {
     int l = 1;

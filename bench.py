@@ -58,15 +58,16 @@ def pair_flops(n_sites, hyp, D, widths):
     return 2 * n_sites * 6 * (nang + 1) * nlm + ffnn_flops_per_pair(D, widths)
 
 
-def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
+def cpu_baseline(spec, shape_name, phi, hyp, net, target_s=15.0):
     """Reference algorithm (10-row FD stencil, FP64 numpy, one thread) on a bounded pair sample."""
     import numpy as np
     from oracle import nep_oracle as O, reference_step as R
     sh = O.shape_table()[shape_name]
     layers = net.layers_f64()
-    # sample: pairs of a 600-particle box at the same density (same per-pair work)
+    # sample: pairs of the workload itself when small, else of a 600-body box at its packing
+    # fraction (same per-pair work)
     from nepx.configs import make_system
-    small = make_system(shape_name, 600, 0.45, seed=123)
+    small = spec if spec.n <= 4096 else make_system(shape_name, 600, phi, seed=123)
     pairs = O.brute_pairs(small.x, small.L, sh["cutoff"])
     from threadpoolctl import threadpool_limits
     rng = np.random.default_rng(0)
@@ -94,7 +95,7 @@ def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
     p_per_particle = len(pairs) / small.n
     step_time = spec.n * (p_per_particle * per_pair + per_particle)
     return {"value": spec.n / step_time, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{done} pairs of a 600-tetrahedron box at the same density through the reference "
+            "sample": f"{done} pairs of a {small.n}-body {shape_name} box at the same density through the reference "
                       f"algorithm (10-row forward-FD stencil, NEP descriptor, FFNN, clip; FP64 numpy, 1 thread) "
                       f"in {t_pairs:.1f} s, scaled by pairs per particle ({p_per_particle:.1f}); FP64 NVT "
                       f"update timed on the full state"}
@@ -298,6 +299,13 @@ def main():
     P = int(flags[3])
     th = ar.thermo()[0]
 
+    import ctypes
+    lanes = int(nepx._lib.lib.nepx_pair_fused_supported(ctypes.byref(ar.model), ctypes.byref(ar.cnet)))
+    kernel = {4: "k_fused_quad (four lanes per pair: descriptor + FFNN fwd/bwd + adjoint)",
+              1: "k_fused (one lane per pair: descriptor + FFNN fwd/bwd + adjoint)"}.get(lanes, "split pair kernels")
+    compute = ("FP32 MFMA v_mfma_f32_4x4x1_16b_f32 (per-pair site contractions) + MFMA (network) + FP32 VALU "
+               "(site geometry, radial functions, harmonics, gradient terms); the bound is FP32 VALU issue"
+               if lanes == 4 else "FP32 VALU (descriptor, adjoint) + MFMA (network); the bound is FP32 compute")
     flops = pair_flops(shape.n_sites, hyp, D, widths) * P
     achieved = flops / (fused_ms * 1e-3) / 1e12
     half = net.precision == "f16x3"
@@ -324,8 +332,7 @@ def main():
                    "list_builds": int(flags[4]), "wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": prof["traffic"],
-                     "kernel": "k_fused (descriptor + FFNN fwd/bwd + adjoint)", "kernel_ms": fused_ms,
-                     "compute": "FP32 VALU (descriptor, adjoint) + MFMA (network); the bound is FP32 compute",
+                     "kernel": kernel, "kernel_ms": fused_ms, "compute": compute,
                      "flops_per_launch": flops, "flops_per_pair": pair_flops(shape.n_sites, hyp, D, widths),
                      "ffnn_flops_per_launch": ffnn_flops_per_pair(D, widths) * P,
                      "algorithmic_bytes_per_launch": P * 72 + spec.n * 16,
@@ -343,7 +350,7 @@ def main():
         torch.cuda.synchronize()
         res["copy_calib_bytes"] = {"read": 4 * n, "write": 4 * n}
     if not a.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(spec, w["shape"], hyp, net)
+        res["cpu_baseline"] = cpu_baseline(spec, w["shape"], w["phi"], hyp, net)
     print(json.dumps(res))
 
 

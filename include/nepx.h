@@ -187,8 +187,9 @@ int nepx_stencil_rows(const float* qf, const int32_t* pair_i, const int32_t* pai
 int nepx_fd_forces(const float* E_rows, int64_t n_pairs, float dx, float dteta, float clip, float* out,
                    void* stream);
 
-/* 1 when nepx_pair_fused has a kernel for this pair model and network, else 0
- * (callers then run nepx_descriptor_fwd -> nepx_ffnn_fwd_bwd -> nepx_descriptor_adjoint). */
+/* Lanes per pair of the kernel nepx_pair_fused runs for this pair model and network under the
+ * current tuning: 4 (site sums on matrix cores) or 1; 0 when there is no fused kernel (callers
+ * then run nepx_descriptor_fwd -> nepx_ffnn_fwd_bwd -> nepx_descriptor_adjoint). */
 int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_net* net);
 /* Fused descriptor -> MFMA network -> adjoint for the step (no G in HBM). */
 int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
