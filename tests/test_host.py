@@ -287,6 +287,29 @@ def test_tuning_roundtrip_and_validation():
         tune.set_launch(tune.Tuning(**old))
 
 
+def test_fused_kernel_lanes_query():
+    """nepx_pair_fused_supported names the fused kernel a model and network get: 4 lanes per pair
+    (site sums on matrix cores) by default where the shape allows, 1 when tuned so, 0 without one."""
+    import ctypes
+    import nepx
+    from nepx import _lib, tune
+    net = _lib.Net()
+    net.d_in, net.width, net.n_hidden, net.act = 40, 64, 1, 0
+    tet = nepx.make_pair_model(nepx.get_shape("tet10"), (11, 6, 4))
+    old = tune.get_launch()
+    try:
+        q = lambda m: _lib.lib.nepx_pair_fused_supported(ctypes.byref(m), ctypes.byref(net))
+        tune.set_launch(tune.Tuning(**{**old, "pair_lanes": 4}))
+        assert q(tet) == 4
+        tune.set_launch(tune.Tuning(**{**old, "pair_lanes": 1}))
+        assert q(tet) == 1
+        ref = nepx.make_pair_model(nepx.get_shape("cube_v2"), (10, 10, 10))   # D = 121: split kernels only
+        net.d_in, net.width, net.n_hidden = 121, 150, 3
+        assert q(ref) == 0
+    finally:
+        tune.set_launch(tune.Tuning(**old))
+
+
 def test_coordinate_descent_finds_separable_minimum(tmp_path):
     from nepx import tune
     target = {"skin": 0.4, "gather_lanes": 8, "fused_blocks_per_cu": 2}
