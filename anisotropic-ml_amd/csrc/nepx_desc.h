@@ -200,12 +200,17 @@ __device__ __forceinline__ SiteGeo site_geo(const nepx_pair_model& m, const Mat3
     return site_geo_b(R, m.sites[3 * k], m.sites[3 * k + 1], m.sites[3 * k + 2], ox, oy, oz, inv_rn);
 }
 
+// cos(pi x) and sin(pi x) on the hardware units (v_cos_f32 / v_sin_f32 take the
+// angle in revolutions, valid for |x| < 512); rho is non-negative and small here.
+__device__ __forceinline__ float cos_pi(float x) { return __builtin_amdgcn_cosf(0.5f * x); }
+__device__ __forceinline__ float sin_pi(float x) { return __builtin_amdgcn_sinf(0.5f * x); }
+
 // Radial functions at rho: fn(n, f_n) for n = 0..NMAX with
 // f_n = (T_n(x) + 1) w fc/2,  x = 2(rho-1)^2 - 1,  fc = (1 + cos(pi rho))/2.
 template <int NMAX, class Fn>
 __device__ __forceinline__ void radial_values(float rho, float w, Fn&& fn) {
     const float x = 2.f * (rho - 1.f) * (rho - 1.f) - 1.f;
-    const float hf = 0.25f * (1.f + cospif(rho)) * w;  // w fc / 2
+    const float hf = 0.25f * (1.f + cos_pi(rho)) * w;  // w fc / 2
     float t0 = 1.f, t1 = x;
 #pragma unroll
     for (int n = 0; n <= NMAX; ++n) {
@@ -227,8 +232,7 @@ template <int NMAX, class Fn>
 __device__ __forceinline__ void radial_derivs(float rho, Fn&& fn) {
     const float xm = rho - 1.f;
     const float x = 2.f * xm * xm - 1.f;
-    float sp, cp;
-    sincospif(rho, &sp, &cp);
+    const float sp = sin_pi(rho), cp = cos_pi(rho);
     const float hf = 0.25f * (1.f + cp);                // fc / 2
     const float hdx = 2.f * xm * (0.5f * (1.f + cp));    // (dx/drho) fc / 2
     const float hdf = -0.25f * (float)M_PI * sp;         // fc' / 2

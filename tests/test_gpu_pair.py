@@ -144,3 +144,26 @@ def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision, pair_la
     print(f"  vs FP64: E {eE:.3e} F {eF:.3e} tau_i {eTi:.3e} tau_j {eTj:.3e}")
     assert eE < 1e-5
     assert max(eF, eTi, eTj) < 1e-4
+
+
+def test_fused_relu_matches_fp64(gpu, pair_lanes):
+    """ReLU networks (the activation of the reference's TheEnergyNet) through both fused kernels."""
+    import nepx
+    from nepx import ops
+    shape, (q, i, j, t), T = _inputs(gpu, "tet10", n_pairs=333, seed=21)
+    hyp = (11, 6, 4)
+    model = nepx.make_pair_model(shape, hyp)
+    net = nepx.EnergyNet.random(40, [64, 64], seed=4, act="relu", en_min=-1.0, en_max=2.0, device=gpu)
+    fused = ops.pair_fused(model, net, T["qf"], T["pi"], T["pj"], T["pt"], clip=0.0).cpu().numpy()
+    clouds = oracle_clouds(q, i, j, t, shape.sites)
+    G64 = O.descriptor(clouds, shape.nep_cutoff, *hyp)
+    y, g = O.ffnn_forward_backward(G64, net.layers_f64(), "relu")
+    gp = O.descriptor_grad(clouds, 3.0 * g, shape.nep_cutoff, *hyp)
+    Ns = shape.n_sites
+    Ti = np.cross(clouds[:, :Ns], -gp[:, :Ns]).sum(1)
+    eE = normwise_rel(fused[:, 3], -1.0 + 3.0 * y)
+    eF = normwise_rel(fused[:, 0:3], -gp[:, :Ns].sum(1))
+    eT = normwise_rel(fused[:, 4:7], Ti)
+    print(f"relu fused ({pair_lanes} lanes/pair) vs FP64: E {eE:.3e} F {eF:.3e} tau_i {eT:.3e}")
+    assert eE < 1e-5
+    assert max(eF, eT) < 1e-4
