@@ -49,7 +49,7 @@ __host__ __device__ constexpr int chunk_split(int d) {
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 // Sum over the four lanes of a quad (quad_perm 1,0,3,2 then 2,3,0,1), the same in every lane.
 __device__ __forceinline__ float quad_sum(float v) {
@@ -110,9 +110,9 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     float* const grow = xw + slot * LD;   // its descriptor row (reuses the exchange area after the forward)
     const float inv_rn = 1.f / m.r_nep;
     constexpr K2Tab<LMAX> k2{};
-    // this lane's accumulator column lm = 4d + q: K_lm^2, and whether lm is in the chunk's upper l
-    float kk[LC];
-    bool up[LC];
+    // this lane's accumulator column lm = 4d + q: K_lm^2 as the weight of the chunk's lower l
+    // (klo) or of its upper l (khi), the other one zero
+    float klo[LC], khi[LC];
 #pragma unroll
     for (int d = 0; d < LC; ++d) {
         float v[4];
@@ -121,8 +121,10 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             const int lm = 4 * d + i;
             v[i] = lm < NLM ? k2(l_of_slot(lm), m_of_slot(lm)) : 0.f;
         }
-        kk[d] = sel4(q, v[0], v[1], v[2], v[3]);
-        up[d] = 4 * d + q >= chunk_split<NLM>(d);
+        const float kk = sel4(q, v[0], v[1], v[2], v[3]);
+        const bool up = 4 * d + q >= chunk_split<NLM>(d);
+        klo[d] = up ? 0.f : kk;
+        khi[d] = up ? kk : 0.f;
     }
 
     for (int base = blockIdx.x * (Q::BLOCK / 4); base < P; base += gridDim.x * (Q::BLOCK / 4)) {
@@ -210,13 +212,9 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
 #pragma unroll
             for (int n = 0; n < NA; ++n) {
                 const float a = Af[n / 4][d][n % 4];
-                const float sq = kk[d] * a * a;
-                if (lo == hi) {
-                    ga[n][lo - 1] += sq;
-                } else {
-                    ga[n][lo - 1] += up[d] ? 0.f : sq;
-                    ga[n][hi - 1] += up[d] ? sq : 0.f;
-                }
+                const float a2 = a * a;
+                ga[n][lo - 1] += klo[d] * a2;
+                if (lo != hi) ga[n][hi - 1] += khi[d] * a2;
             }
         }
 #pragma unroll
@@ -224,13 +222,13 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
 #pragma unroll
             for (int l = 0; l < LMAX; ++l) ga[n][l] = quad_sum(ga[n][l]);
         wave_lds_order();  // the exchange reads are done before the rows overwrite the area
-        if (q == 0) {
+        if (q == 0) {  // rows of pairs past the end hold finite values of the stand-in pair; unused
 #pragma unroll
-            for (int n = 0; n <= Cfg::NRAD; ++n) grow[n] = valid ? gr[n] : 0.f;
+            for (int n = 0; n <= Cfg::NRAD; ++n) grow[n] = gr[n];
 #pragma unroll
             for (int n = 0; n < NA; ++n)
 #pragma unroll
-                for (int l = 0; l < LMAX; ++l) grow[GA + n * LMAX + l] = valid ? ga[n][l] : 0.f;
+                for (int l = 0; l < LMAX; ++l) grow[GA + n * LMAX + l] = ga[n][l];
         }
         wave_lds_order();
         // ---- energy network on the wave's 16 pairs: rows in, dE/dG rows out ----
@@ -256,8 +254,9 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             const int lo = l_of_slot(4 * d), hi = l_of_slot(4 * d + 3 < NLM ? 4 * d + 3 : NLM - 1);
 #pragma unroll
             for (int n = 0; n < NA; ++n) {
-                const float wlo = grow[GA + n * LMAX + lo - 1], whi = grow[GA + n * LMAX + hi - 1];
-                Af[n / 4][d][n % 4] *= 2.f * kk[d] * (up[d] ? whi : wlo);
+                const float w = klo[d] * grow[GA + n * LMAX + lo - 1] +
+                                (lo != hi ? khi[d] * grow[GA + n * LMAX + hi - 1] : 0.f);
+                Af[n / 4][d][n % 4] *= 2.f * w;
             }
         }
         // per-lane sums over its sites on body i and on body j
