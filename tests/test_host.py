@@ -166,6 +166,137 @@ def test_half_net_packing_and_precision(d_in, width, n_hidden, act):
     assert np.abs(dG - g64).max() < 1e-4 * np.abs(g64).max()
 
 
+def _quad_layout(nang, lmax, n_sites, f, Z, B):
+    """Emulate the four-lanes-per-pair kernel's data movement for one pair (csrc/nepx_quad.h):
+    the LDS exchange xp[16 ch + 4 i + q], the reads of element q of every chunk, and
+    v_mfma_f32_4x4x1_16b_f32 as D[i][j] += a(lane i) b(lane j) held by lane j, register i.
+    f: (n_sites, nang+1), Z: (n_sites, NLM) per-site values; B: (nang+1, NLM) adjoint weights.
+    Returns A[n][lm] as lanes hold it after the forward, the g_ang row from the per-lane l weights
+    and quad sums, and c_lm per site from the adjoint products."""
+    from math import factorial
+    NA, NLM = nang + 1, lmax * (lmax + 2)
+    NC, LC = (NA + 3) // 4, (NLM + 3) // 4
+    def l_of(lm):
+        l = 1
+        while (l + 1) ** 2 - 1 <= lm:
+            l += 1
+        return l
+    def m_of(lm):
+        return (lm - (l_of(lm) ** 2 - 1) + 1) // 2
+    def split(d):
+        lo, hi = l_of(4 * d), l_of(min(4 * d + 3, NLM - 1))
+        return 4 * d + 4 if hi == lo else hi * hi - 1
+    k2 = lambda l, m: 1.0 if m == 0 else 2.0 * factorial(l - m) / factorial(l + m)
+    # forward: Af[c][d][lane j][register i]
+    Af = np.zeros((NC, LC, 4, 4))
+    for s0 in range(0, n_sites, 4):
+        vals = np.zeros((4, 16 * (NC + LC)))
+        for q in range(4):                                   # lane q computes site s0 + q
+            fv = np.zeros(4 * NC); fv[:NA] = f[s0 + q]
+            zv = np.zeros(4 * LC); zv[:NLM] = Z[s0 + q]
+            v = np.concatenate([fv, zv])
+            for val_idx, x in enumerate(v):                  # value 4 ch + i -> xp[16 ch + 4 i + q]
+                ch, i = divmod(val_idx, 4)
+                vals[0, 16 * ch + 4 * i + q] = x
+        xp = vals[0]
+        xf = np.array([[xp[16 * c + 4 * L: 16 * c + 4 * L + 4] for c in range(NC)] for L in range(4)])
+        xz = np.array([[xp[16 * (NC + d) + 4 * L: 16 * (NC + d) + 4 * L + 4] for d in range(LC)] for L in range(4)])
+        for k in range(4):
+            for c in range(NC):
+                for d in range(LC):
+                    for i in range(4):
+                        for j in range(4):
+                            Af[c, d, j, i] += xf[i, c, k] * xz[j, d, k]
+    A = np.zeros((4 * NC, 4 * LC))
+    for c in range(NC):
+        for d in range(LC):
+            for j in range(4):
+                for i in range(4):
+                    A[4 * c + i, 4 * d + j] = Af[c, d, j, i]
+    # g_ang from per-lane weights and quad sums
+    ga = np.zeros((NA, lmax))
+    for j in range(4):
+        for d in range(LC):
+            lm = 4 * d + j
+            kk = k2(l_of(lm), m_of(lm)) if lm < NLM else 0.0
+            up = lm >= split(d)
+            lo, hi = l_of(4 * d), l_of(min(4 * d + 3, NLM - 1))
+            for n in range(NA):
+                a2 = Af[n // 4, d, j, n % 4] ** 2
+                ga[n, lo - 1] += (0.0 if up else kk) * a2
+                if lo != hi:
+                    ga[n, hi - 1] += (kk if up else 0.0) * a2
+    # adjoint: Bf in the forward layout, c^T = B^T f per step of four sites
+    Bf = np.zeros((NC, LC, 4, 4))
+    for n in range(NA):
+        for lm in range(NLM):
+            Bf[n // 4, lm // 4, lm % 4, n % 4] = B[n, lm]
+    cs = np.zeros((n_sites, 4 * LC))
+    for s0 in range(0, n_sites, 4):
+        cc = np.zeros((LC, 4, 4))                            # [d][lane j][register i]
+        for n in range(NA):
+            for d in range(LC):
+                for i in range(4):
+                    for j in range(4):
+                        cc[d, j, i] += Bf[n // 4, d, i, n % 4] * f[s0 + j, n]
+        for j in range(4):
+            for d in range(LC):
+                cs[s0 + j, 4 * d: 4 * d + 4] = cc[d, j]
+    return A[:NA, :NLM], ga, cs[:, :NLM]
+
+
+@pytest.mark.parametrize("nang,lmax,n_sites", [(6, 4, 20), (6, 6, 24), (8, 4, 12), (10, 10, 8)])
+def test_quad_kernel_layout_emulation(nang, lmax, n_sites):
+    """The four-lane kernel's exchange and 4x4 block products give A = f^T Z, the descriptor's
+    g_ang[n][l] = sum_m K_lm^2 A_nlm^2, and c = B^T f per site (checked against the dense forms
+    and the FP64 oracle's angular descriptor)."""
+    from math import factorial
+    rng = np.random.default_rng(nang * 100 + lmax)
+    NA, NLM = nang + 1, lmax * (lmax + 2)
+    f = rng.standard_normal((n_sites, NA))
+    Z = rng.standard_normal((n_sites, NLM))
+    B = rng.standard_normal((NA, NLM))
+    A, ga, cs = _quad_layout(nang, lmax, n_sites, f, Z, B)
+    assert np.allclose(A, f.T @ Z, rtol=1e-12, atol=1e-12)
+    want = np.zeros((NA, lmax))
+    for l in range(1, lmax + 1):
+        for m in range(0, l + 1):
+            k2 = 1.0 if m == 0 else 2.0 * factorial(l - m) / factorial(l + m)
+            for part in range(2 if m else 1):
+                lm = l * l - 1 + (0 if m == 0 else 2 * m - 1 + part)
+                want[:, l - 1] += k2 * A[:, lm] ** 2
+    assert np.allclose(ga, want, rtol=1e-12)
+    assert np.allclose(cs, f @ B, rtol=1e-12, atol=1e-12)
+
+
+def test_quad_layout_g_ang_is_the_descriptor():
+    """With f and Z of a real site cloud the emulated g_ang equals the oracle's angular descriptor."""
+    rng = np.random.default_rng(5)
+    nrad, nang, lmax, rc = 11, 6, 4, 2.5
+    cloud = rng.uniform(-1.2, 1.2, (1, 20, 3))
+    G = O.descriptor(cloud, rc, nrad, nang, lmax)[0]
+    r = cloud[0] - cloud[0].mean(0)
+    rho = np.linalg.norm(r, axis=1) / rc
+    u = r / np.linalg.norm(r, axis=1, keepdims=True)
+    from scipy.special import eval_chebyt, lpmv
+    x = 2 * (rho - 1) ** 2 - 1
+    fc = 0.5 * (1 + np.cos(np.pi * rho))
+    f = np.stack([(eval_chebyt(n, x) + 1) * 0.5 * fc for n in range(nang + 1)], 1)
+    # unnormalised real solid harmonics on the unit sphere: Q_l^m(z) (Condon-Shortley) x Re/Im (x+iy)^m
+    NLM = lmax * (lmax + 2)
+    Z = np.zeros((20, NLM))
+    phi = np.arctan2(u[:, 1], u[:, 0])
+    st = np.sqrt(u[:, 0] ** 2 + u[:, 1] ** 2)
+    for l in range(1, lmax + 1):
+        for m in range(0, l + 1):
+            Q = lpmv(m, l, u[:, 2]) / np.where(st > 0, st, 1.0) ** m if m else lpmv(0, l, u[:, 2])
+            for part in range(2 if m else 1):
+                lm = l * l - 1 + (0 if m == 0 else 2 * m - 1 + part)
+                Z[:, lm] = Q * (st ** m) * (np.sin(m * phi) if part else np.cos(m * phi))
+    _, ga, _ = _quad_layout(nang, lmax, 20, f, Z, np.zeros((nang + 1, NLM)))
+    assert np.allclose(ga.ravel(), G[nrad + 1:], rtol=1e-9, atol=1e-12)
+
+
 def test_half_net_needs_width_multiple_of_32():
     import nepx
     from nepx import _lib
