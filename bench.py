@@ -155,6 +155,60 @@ def serve_bench(a):
     return res
 
 
+def ffnn_bench(a):
+    """The energy network alone (nepx_ffnn_fwd_bwd: forward and backward to the input) on the
+    descriptor rows of one 32,768-tetrahedron step: 4,052,090 rows of D = 40 through
+    40 -> 64 -> 64 -> 1 tanh, inputs and outputs resident in HBM."""
+    import ctypes
+    import torch
+    import nepx
+    from nepx import _lib
+    dev = torch.device("cuda:0")
+    P, D, widths = 4052090, 40, [64, 64]
+    gen = torch.Generator().manual_seed(11)
+    G = (torch.rand((P, D), generator=gen) * 4.0).to(dev)        # descriptor-like magnitudes
+    net = nepx.EnergyNet.random(D, widths, seed=7, device=dev, precision=a.precision)
+    E = torch.empty(P, dtype=torch.float32, device=dev)
+    dG = torch.empty_like(G)
+    n_dev = torch.tensor([P], dtype=torch.int32, device=dev)
+    cn = net.cstruct()
+    stream = torch.cuda.current_stream()
+    h = _lib.stream_handle(stream)
+
+    def run():
+        _lib.check(_lib.lib.nepx_ffnn_fwd_bwd(ctypes.byref(cn), _lib.ptr(G), _lib.ptr(n_dev), P, _lib.ptr(E),
+                                              _lib.ptr(dG), h), "nepx_ffnn_fwd_bwd")
+    for _ in range(a.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(a.steps):
+        run()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / a.steps
+    flops = ffnn_flops_per_pair(D, widths) * P                    # FP32-equivalent, unpadded
+    half = net.precision == "f16x3"
+    # fp16 MFMAs the split network issues per 16-row tile (3 per 32-deep k-step): forward
+    # 64-padded input and 64-wide layer, backward through both (dE/dG to 48 padded inputs)
+    hw_f16 = (P / 16) * 90 * 16 * 16 * 32 * 2 if half else None
+    return {
+        "metric": "energy-network rows/s (forward + backward to the descriptor, 32k-tetrahedron step)",
+        "value": P / (ms * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (fp16 hi/lo split MFMA, FP32 accumulate)" if half else "f32 (MFMA f32)",
+        "data": "synthetic (uniform [0, 4) descriptors, random-init FFNN)",
+        "config": {"workload": f"{P} rows, FFNN {D}->{widths[0]}->{widths[1]}->1 tanh, nepx_ffnn_fwd_bwd"},
+        "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP32_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / 1e12 / PEAK_FP32_MFMA_TFLOPS,
+                     "traffic": None, "kernel": "k_ffnn", "flops_per_launch": flops,
+                     "fp16_mfma_tflops": None if hw_f16 is None else hw_f16 / (ms * 1e-3) / 1e12,
+                     "fp16_peak_tflops": PEAK_FP16_MFMA_TFLOPS,
+                     "algorithmic_bytes_per_launch": P * (2 * 4 * D + 4)},
+    }
+
+
 def replicas_bench(a):
     """256 independent cube systems advanced by the same launches (one arena, hipGraph steps) against
     the same systems run one at a time with the single-system step."""
@@ -246,6 +300,9 @@ def main():
 
     if a.workload == "serve":
         print(json.dumps(serve_bench(a)))
+        return
+    if a.workload == "ffnn":
+        print(json.dumps(ffnn_bench(a)))
         return
     if a.workload == "replicas256":
         print(json.dumps(replicas_bench(a)))
