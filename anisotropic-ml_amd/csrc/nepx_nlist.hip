@@ -27,12 +27,18 @@ __device__ __forceinline__ double wrap1(double t, double L) {
     return t;
 }
 
+// Squared minimum-image distance from (xi, yi, zi) to (xj, yj, zj); t = X_j - X_i.
+__device__ __forceinline__ double pos_d2(double xi, double yi, double zi, double xj, double yj, double zj, double L,
+                                         double& tx, double& ty, double& tz) {
+    tx = wrap1(__dsub_rn(xj, xi), L);
+    ty = wrap1(__dsub_rn(yj, yi), L);
+    tz = wrap1(__dsub_rn(zj, zi), L);
+    return __dadd_rn(__dadd_rn(__dmul_rn(tx, tx), __dmul_rn(ty, ty)), __dmul_rn(tz, tz));
+}
+
 __device__ __forceinline__ double pair_d2(const nepx_state& st, double L, int i, int j, double& tx,
                                           double& ty, double& tz) {
-    tx = wrap1(__dsub_rn(st.x[j], st.x[i]), L);
-    ty = wrap1(__dsub_rn(st.y[j], st.y[i]), L);
-    tz = wrap1(__dsub_rn(st.z[j], st.z[i]), L);
-    return __dadd_rn(__dadd_rn(__dmul_rn(tx, tx), __dmul_rn(ty, ty)), __dmul_rn(tz, tz));
+    return pos_d2(st.x[i], st.y[i], st.z[i], st.x[j], st.y[j], st.z[j], L, tx, ty, tz);
 }
 
 __device__ __forceinline__ int cell_coord(double x, double L, int nc) {
@@ -111,6 +117,8 @@ __global__ void k_cells_fill(nepx_state st, nepx_nlist nl) {
     const int c = nl.cell_of[i];
     const int pos = nl.cell_start[c] + atomicAdd(&nl.cell_fill[c], 1);
     nl.cell_list[pos] = i;
+    // positions in cell order: the candidate walk reads a cell's positions contiguously
+    reinterpret_cast<double4*>(nl.cell_pos)[pos] = make_double4(st.x[i], st.y[i], st.z[i], 0.0);
 }
 
 // The list build gives each particle a group of kLG lanes.  The lanes walk
@@ -118,8 +126,10 @@ __global__ void k_cells_fill(nepx_state st, nepx_nlist nl) {
 // box is too small for three cells a side) in rounds of kLG: round r of a
 // cell covers its entries r*kLG + sub.  Every lane of a group sees the same
 // cells and rounds (same i), so ballots inside a round are uniform over the
-// group; f(j, valid) is called for each round slot, with valid = false for
-// slots past the end of a cell and for j == i.
+// group; f(j, valid, d2, tx, ty, tz) is called for each round slot with the
+// squared minimum-image distance and t = X_j - X_i, valid = false for slots
+// past the end of a cell and for j == i.  Cell candidates' positions come from
+// the cell-ordered copy (contiguous per cell), the same FP64 values as st.x.
 constexpr int kLG = 16;
 
 __device__ __forceinline__ uint32_t group_mask(uint64_t ballot) {
@@ -132,14 +142,19 @@ __device__ __forceinline__ void for_candidates(const nepx_state& st, const nepx_
     const int s = st.sys[i];
     const nepx_sysparam& sp = st.sp[s];
     const double L = sp.L;
+    const double xi = st.x[i], yi = st.y[i], zi = st.z[i];
+    double tx, ty, tz;
     if (sp.nc <= 0) {
         for (int base = sp.p_start; base < sp.p_end; base += kLG) {
             const int j = base + sub;
             const bool valid = j < sp.p_end && j != i && st.sys[j] == s;
-            f(valid ? j : i, valid);
+            const int jj = valid ? j : i;
+            const double d2 = pos_d2(xi, yi, zi, st.x[jj], st.y[jj], st.z[jj], L, tx, ty, tz);
+            f(jj, valid, d2);
         }
         return;
     }
+    const double4* cpos = reinterpret_cast<const double4*>(nl.cell_pos);
     const int nc = sp.nc;
     const int cx = cell_coord(st.x[i], L, nc), cy = cell_coord(st.y[i], L, nc), cz = cell_coord(st.z[i], L, nc);
     for (int dx = -1; dx <= 1; ++dx) {
@@ -152,8 +167,11 @@ __device__ __forceinline__ void for_candidates(const nepx_state& st, const nepx_
                 const int b = nl.cell_start[c], e = nl.cell_start[c + 1];
                 for (int base = b; base < e; base += kLG) {
                     const int k = base + sub;
-                    const int j = k < e ? nl.cell_list[k] : i;
-                    f(j, j != i);
+                    const bool in = k < e;
+                    const int j = in ? nl.cell_list[k] : i;
+                    const double4 pj = in ? cpos[k] : make_double4(xi, yi, zi, 0.0);
+                    const double d2 = pos_d2(xi, yi, zi, pj.x, pj.y, pj.z, L, tx, ty, tz);
+                    f(j, j != i, d2);
                 }
             }
         }
@@ -176,9 +194,8 @@ __global__ __launch_bounds__(kNB) void k_nbr_count(nepx_state st, nepx_nlist nl)
     if (live(st, i)) {
         const nepx_sysparam& sp = st.sp[st.sys[i]];
         const double r2 = __dmul_rn(sp.r_list, sp.r_list);
-        for_candidates(st, nl, i, sub, [&](int j, bool valid) {
-            double tx, ty, tz;
-            if (valid && pair_d2(st, sp.L, i, j, tx, ty, tz) < r2) {
+        for_candidates(st, nl, i, sub, [&](int j, bool valid, double d2) {
+            if (valid && d2 < r2) {
                 if (j > i) ++nf;
                 else ++nr;
             }
@@ -219,9 +236,8 @@ __global__ __launch_bounds__(kNB) void k_nbr_fill(nepx_state st, nepx_nlist nl) 
     const double r2 = __dmul_rn(sp.r_list, sp.r_list);
     const uint32_t below = (1u << sub) - 1u;
     int pf = nl.row_start[i], pr = nl.rrow_start[i];
-    for_candidates(st, nl, i, sub, [&](int j, bool valid) {
-        double tx, ty, tz;
-        const bool in = valid && pair_d2(st, sp.L, i, j, tx, ty, tz) < r2;
+    for_candidates(st, nl, i, sub, [&](int j, bool valid, double d2) {
+        const bool in = valid && d2 < r2;
         const bool fwd = in && j > i, rev = in && j < i;
         const uint32_t mf = group_mask(__ballot(fwd)), mr = group_mask(__ballot(rev));
         if (fwd) {
@@ -553,8 +569,9 @@ using namespace nepx;
 extern "C" int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, void* stream) {
     int rc = check_state(st, nl);
     if (rc) return rc;
-    NEPX_REQUIRE(nl->cell_count && nl->cell_start && nl->cell_fill && nl->cell_list && nl->cell_of, NEPX_EINVAL,
-                 "null cell buffer");
+    NEPX_REQUIRE(nl->cell_count && nl->cell_start && nl->cell_fill && nl->cell_list && nl->cell_of && nl->cell_pos,
+                 NEPX_EINVAL, "null cell buffer");
+    NEPX_REQUIRE(((uintptr_t)nl->cell_pos & 31) == 0, NEPX_EINVAL, "cell_pos must be 32-byte aligned");
     hipStream_t s = as_stream(stream);
     const int n = st->n;
     const int gb = blocks_for(n, kNB);

@@ -42,7 +42,7 @@ class State(C.Structure):
     _fields_ = [("n", C.c_int32), ("n_sys", C.c_int32)] + [(k, C.c_void_p) for k in _STATE_PTRS]
 
 
-_NL_PTRS = ["cell_count", "cell_start", "cell_fill", "cell_list", "cell_of", "row_count", "row_start",
+_NL_PTRS = ["cell_count", "cell_start", "cell_fill", "cell_list", "cell_pos", "cell_of", "row_count", "row_start",
             "rrow_count", "rrow_start", "nbr", "slot_row", "rnbr", "cidx", "blk_count", "flags", "pair_i",
             "pair_j", "pair_t"]
 

@@ -162,6 +162,7 @@ class Arena:
         L = {
             "cell_count": i32(self.n_cells + 1), "cell_start": i32(self.n_cells + 1),
             "cell_fill": i32(self.n_cells + 1), "cell_list": i32(n), "cell_of": i32(n),
+            "cell_pos": torch.zeros(4 * n, dtype=torch.float64, device=dev),
             "row_count": i32(n + 1), "row_start": i32(n + 1), "rrow_count": i32(n + 1),
             "rrow_start": i32(n + 1), "nbr": i32(cap), "slot_row": i32(cap), "rnbr": i32(cap),
             "cidx": i32(cap), "blk_count": i32(cap // 1024 + 4), "flags": i32(_lib.N_FLAGS),

@@ -104,6 +104,7 @@ typedef struct nepx_nlist {
     int32_t* cell_start;             /* n_cells_total+1 */
     int32_t* cell_fill;              /* n_cells_total   */
     int32_t* cell_list;              /* n               */
+    double* cell_pos;                /* 4n: x, y, z of cell_list[k] at 4k..4k+2 (written by the build) */
     int32_t* cell_of;                /* n               */
     int32_t* row_count;              /* n+1 (forward), then scanned to row_start  */
     int32_t* row_start;              /* n+1 */
