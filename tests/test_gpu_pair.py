@@ -146,6 +146,22 @@ def test_fused_matches_pipeline(gpu, shape_name, hyp, widths, precision, pair_la
     assert max(eF, eTi, eTj) < 1e-4
 
 
+def test_fused_result_independent_of_list_position(gpu, pair_lanes):
+    """Each pair's output is bitwise the same wherever the pair sits in the list: the 4x4 block
+    products, quad sums and 16-pair network tiles never mix pairs."""
+    import nepx
+    from nepx import ops
+    shape, (q, i, j, t), T = _inputs(gpu, "tet10", n_pairs=517, seed=31)
+    model = nepx.make_pair_model(shape, (11, 6, 4))
+    net = nepx.EnergyNet.random(40, [64, 64], seed=6, device=gpu)
+    out = ops.pair_fused(model, net, T["qf"], T["pi"], T["pj"], T["pt"], clip=100.0).cpu().numpy()
+    perm = np.random.default_rng(0).permutation(len(i))
+    pt = torch.from_numpy(np.ascontiguousarray(T["pt"].cpu().numpy()[perm])).to(gpu)
+    out_p = ops.pair_fused(model, net, T["qf"], T["pi"][torch.from_numpy(perm).to(gpu)].contiguous(),
+                           T["pj"][torch.from_numpy(perm).to(gpu)].contiguous(), pt, clip=100.0).cpu().numpy()
+    assert np.array_equal(out_p, out[perm])
+
+
 def test_fused_relu_matches_fp64(gpu, pair_lanes):
     """ReLU networks (the activation of the reference's TheEnergyNet) through both fused kernels."""
     import nepx
