@@ -58,6 +58,16 @@ __global__ void k_set_flag(int32_t* flags, int idx, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) flags[idx] = v;
 }
 
+// Start of a build step: clear the longest-row flag and, when a rebuild is due, the cell counters.
+__global__ void k_build_begin(int32_t* flags, int32_t* cell_count, int32_t* cell_fill, int64_t nct) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) flags[F_MAXROW] = 0;
+    if (!want_build(flags)) return;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nct; k += (int64_t)gridDim.x * blockDim.x) {
+        cell_count[k] = 0;
+        cell_fill[k] = 0;
+    }
+}
+
 __global__ void k_zero_i32(int32_t* a, int64_t n, const int32_t* flags) {
     if (flags && !want_build(flags)) return;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
@@ -67,7 +77,12 @@ __global__ void k_zero_i32(int32_t* a, int64_t n, const int32_t* flags) {
 __global__ void k_cells_count(nepx_state st, nepx_nlist nl) {
     if (!want_build(nl.flags)) return;
     const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i >= st.n || !live(st, i)) return;
+    if (i >= st.n) return;
+    // positions at this build: the reference for the displacement test that requests the next one
+    st.xb[i] = st.x[i];
+    st.yb[i] = st.y[i];
+    st.zb[i] = st.z[i];
+    if (!live(st, i)) return;
     const int s = st.sys[i];
     const nepx_sysparam& sp = st.sp[s];
     if (sp.nc <= 0) return;
@@ -252,11 +267,12 @@ __global__ __launch_bounds__(kNB) void k_nbr_fill(nepx_state st, nepx_nlist nl) 
 }
 
 // Sort each row ascending: one wave per row (blocks of 64 stride over the
-// rows), bitonic in LDS.  The grid is capped, so the launch costs little when
-// no rebuild is due.
+// rows; grid y selects the forward or the reverse rows), bitonic in LDS.  The
+// grid is capped, so the launch costs little when no rebuild is due.
 constexpr int kSortThreads = 64;
-__global__ __launch_bounds__(kSortThreads) void k_sort_rows(nepx_nlist nl, int n, int reverse) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_rows(nepx_nlist nl, int n) {
     if (!want_build(nl.flags) || nl.flags[F_OVERFLOW]) return;
+    const int reverse = blockIdx.y;  // y = 0: forward rows, 1: reverse rows
     __shared__ int32_t buf[kSortMax];
     const int32_t* start = reverse ? nl.rrow_start : nl.row_start;
     int32_t* data = reverse ? nl.rnbr : nl.nbr;
@@ -310,16 +326,6 @@ __global__ __launch_bounds__(kNB) void k_rev_slots(nepx_nlist nl, int n) {
         }
         if (slot < 0) nl.flags[F_OVERFLOW] = 2;  // inconsistent list: never expected
         nl.rnbr[e] = slot;
-    }
-}
-
-__global__ void k_build_done(nepx_state st, nepx_nlist nl) {
-    if (!want_build(nl.flags)) return;
-    const int i = blockIdx.x * kNB + threadIdx.x;
-    if (i < st.n) {
-        st.xb[i] = st.x[i];
-        st.yb[i] = st.y[i];
-        st.zb[i] = st.z[i];
     }
 }
 
@@ -580,12 +586,9 @@ extern "C" int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, v
         hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_OVERFLOW, 0);
         NEPX_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_MAXROW, 0);
     const int nct = nl->n_cells_total > 0 ? nl->n_cells_total : 1;
-    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(nct, kNB)), dim3(kNB), 0, s, nl->cell_count, (int64_t)nct,
-                       nl->flags);
-    hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(nct, kNB)), dim3(kNB), 0, s, nl->cell_fill, (int64_t)nct,
-                       nl->flags);
+    hipLaunchKernelGGL(k_build_begin, dim3(blocks_for(nct, kNB)), dim3(kNB), 0, s, nl->flags, nl->cell_count,
+                       nl->cell_fill, (int64_t)nct);
     hipLaunchKernelGGL(k_cells_count, dim3(gb), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(kScanBlock), 0, s, *nl);
     hipLaunchKernelGGL(k_cells_fill, dim3(gb), dim3(kNB), 0, s, *st, *nl);
@@ -594,10 +597,8 @@ extern "C" int nepx_nlist_build(nepx_state* st, nepx_nlist* nl, int32_t force, v
     hipLaunchKernelGGL(k_scan_rows, dim3(2), dim3(kScanBlock), 0, s, *nl, n);
     hipLaunchKernelGGL(k_nbr_fill, dim3(gg), dim3(kNB), 0, s, *st, *nl);
     const int gs = n < 16384 ? n : 16384;
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 0);
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 1);
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs, 2), dim3(kSortThreads), 0, s, *nl, n);
     hipLaunchKernelGGL(k_rev_slots, dim3(gg), dim3(kNB), 0, s, *nl, n);
-    hipLaunchKernelGGL(k_build_done, dim3(gb), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
@@ -634,8 +635,7 @@ extern "C" int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64
         hipLaunchKernelGGL(k_pairs_fill, dim3(blocks_for(m, kNB)), dim3(kNB), 0, s, *nl, pairs, m,
                            nl->cell_fill, nl->cell_fill + n);
     const int gs = n < 16384 ? n : 16384;
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 0);
-    hipLaunchKernelGGL(k_sort_rows, dim3(gs), dim3(kSortThreads), 0, s, *nl, n, 1);
+    hipLaunchKernelGGL(k_sort_rows, dim3(gs, 2), dim3(kSortThreads), 0, s, *nl, n);
     hipLaunchKernelGGL(k_rev_slots, dim3(blocks_for((int64_t)n * kLG, kNB)), dim3(kNB), 0, s, *nl, n);
     hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(64), 0, s, *nl);
     NEPX_LAUNCH_CHECK();
