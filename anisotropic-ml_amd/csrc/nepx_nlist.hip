@@ -141,9 +141,9 @@ __global__ void k_cells_fill(nepx_state st, nepx_nlist nl) {
 // box is too small for three cells a side) in rounds of kLG: round r of a
 // cell covers its entries r*kLG + sub.  Every lane of a group sees the same
 // cells and rounds (same i), so ballots inside a round are uniform over the
-// group; f(j, valid, d2, tx, ty, tz) is called for each round slot with the
-// squared minimum-image distance and t = X_j - X_i, valid = false for slots
-// past the end of a cell and for j == i.  Cell candidates' positions come from
+// group; f(j, valid, d2) is called for each round slot with the squared
+// minimum-image distance d2, valid = false for slots past the end of a cell
+// and for j == i.  Cell candidates' positions come from
 // the cell-ordered copy (contiguous per cell), the same FP64 values as st.x.
 constexpr int kLG = 16;
 
