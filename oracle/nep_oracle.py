@@ -29,7 +29,13 @@ eval_legendre (descriptor bases), the spherical-harmonic addition theorem
 (angular descriptor), brute force vs scipy cKDTree (pair sets), torch
 nn.Linear in float64 (FFNN), torch autograd + central finite differences
 (adjoint forces and torques), closed-form free-particle / spherical-top
-motion and time reversibility (integrator), site multiplicity (type
+motion and time reversibility (integrator); with forces, torques and the
+thermostat on (tests/test_oracle_integrator.py): uniform acceleration under a
+constant force, lab angular momentum growing by dt*tau per step under a
+constant torque, the continuous friction/force steady state to the splitting
+error, and second-order conservation of the NVE energy and of the
+Nose-Hoover extended energy, with planted mistakes shown to fail them
+(integrator kicks, torque_dp, thermo_update), site multiplicity (type
 weights), scipy Euler angles, a closed form and finite differences (label
 potential).  No reference-held golden vectors exist for this path (the
 reference has no tests); its data files (pair clouds, initial configuration,
@@ -566,33 +572,46 @@ def rot_ke(q, p, moi):
     return 0.5 * np.sum(s[..., 1:] ** 2 / moi, axis=-1)
 
 
+def nvt_step_two(state, F, T, dt, thermo):
+    """integrate_step_two after the thermostat update (NVT.py:36-64, IntegrateStepTwoKernels.py:22-66):
+    v = v*theta + dt/2 F/m, p = p*e_r + dp with dp = dt q (x) (0, R(q*) tau).
+
+    Returns (state, dp); the state holds the on-step velocity and angular momentum.
+    """
+    x, v, q, p, m = (np.array(state[k], dtype=np.float64, copy=True) for k in "xvqpm")
+    v = v * thermo["theta"] + 0.5 * dt * (F / m[:, None])
+    dp = torque_dp(q, T, dt)
+    p = p * thermo["e_r"] + dp
+    return dict(x=x, v=v, q=q, p=p, m=m), dp
+
+
+def nvt_step_one(state, F, dp, dt, L, moi, thermo):
+    """integrate_step_one (NVT.py:12-15,67-129, IntegrateStepOneKernels.py:3-251):
+    v = (v + dt/2 F/m)*theta, x += v dt (one wrap), p = (p + dp)*e_r, NO_SQUISH rotor, q
+    renormalised; returns (state, Kt, Kr) with the kinetic energies of the new state."""
+    x, v, q, p, m = (np.array(state[k], dtype=np.float64, copy=True) for k in "xvqpm")
+    v = (v + 0.5 * dt * (F / m[:, None])) * thermo["theta"]
+    x = x + v * dt
+    x = np.where(x > 0.5 * L, x - L, x)
+    x = np.where(x < -0.5 * L, x + L, x)
+    p = (p + dp) * thermo["e_r"]
+    p, q = no_squish(p, q, moi, dt)
+    q = q / np.linalg.norm(q, axis=-1, keepdims=True)
+    Kt = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
+    Kr = float(np.sum(rot_ke(q, p, moi)))
+    return dict(x=x, v=v, q=q, p=p, m=m), Kt, Kr
+
+
 def nvt_step(state, F, T, dt, kT, tau, L, moi, thermo):
-    """One integrate_step_two + integrate_step_one with given F, T (App. A.4).
+    """One integrate_step_two + integrate_step_one with given F, T (Sim.py:276-282).
 
     state: dict x, v, q, p, m (FP64 arrays).  thermo: dict xi, eta, xi_r,
     eta_r, theta, e_r already updated from the previous kinetic energies.
     Returns (new_state, Kt, Kr) -- the kinetic energies feed the next
     thermo_update.
     """
-    x, v, q, p, m = (np.array(state[k], dtype=np.float64, copy=True) for k in "xvqpm")
-    th, er = thermo["theta"], thermo["e_r"]
-    a = F / m[:, None]
-    # step two: translation half kick and torque half kick
-    v = v * th + 0.5 * dt * a
-    dp = torque_dp(q, T, dt)
-    p = p * er + dp
-    # step one: translation (IntegrateStepOneKernels.py:202-207)
-    v = (v + 0.5 * dt * a) * th
-    x = x + v * dt
-    x = np.where(x > 0.5 * L, x - L, x)
-    x = np.where(x < -0.5 * L, x + L, x)
-    # step one: rotation
-    p = (p + dp) * er
-    p, q = no_squish(p, q, moi, dt)
-    q = q / np.linalg.norm(q, axis=-1, keepdims=True)
-    Kt = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
-    Kr = float(np.sum(rot_ke(q, p, moi)))
-    return dict(x=x, v=v, q=q, p=p, m=m), Kt, Kr
+    mid, dp = nvt_step_two(state, F, T, dt, thermo)
+    return nvt_step_one(mid, F, dp, dt, L, moi, thermo)
 
 
 # ---------------------------------------------------------------------------
