@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBeadBlock) void k_bead_pairs(const float* __restri
         tjy = wave_sum(tjy);
         tjz = wave_sum(tjz);
         if (valid && lane == 0) {
-            auto cl = [clip](float v) { return clip > 0.f ? fminf(fmaxf(v, -clip), clip) : v; };
+            auto cl = [clip](float v) { return clipf(v, clip); };
             float4* o4 = reinterpret_cast<float4*>(out) + 3 * p;
             o4[0] = make_float4(cl(fx), cl(fy), cl(fz), e);
             o4[1] = make_float4(cl(tix), cl(tiy), cl(tiz), 0.f);

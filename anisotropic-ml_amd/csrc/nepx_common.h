@@ -50,6 +50,14 @@ constexpr double cfact(int n) {
     return r;
 }
 
+// ---- per-component force / torque clip -------------------------------------
+// v > c -> c, v < -c -> -c, anything else (including NaN) unchanged: the
+// masked assignments of ForceEvaluator.py:125-131.  c <= 0 disables the clip.
+__device__ __forceinline__ float clipf(float v, float c) {
+    if (!(c > 0.f)) return v;
+    return v > c ? c : (v < -c ? -c : v);
+}
+
 // ---- quaternion helpers (float, device) ------------------------------------
 // R(q) v = (w^2-|u|^2) v + 2 w (u x v) + 2 (u.v) u  as a 3x3 matrix, written
 // out so that it equals the reference formula (PosToPtsKernels.py:54-71) for

@@ -139,12 +139,14 @@ __device__ __forceinline__ void for_passes(Fn&& fn) {
 }
 
 // One pair in the frame centred on particle i: orientations, t = X_j - X_i,
-// and the centroid of the 2*Ns sites,
-//   c = (R(q_i) sum_k b_k + R(q_j) sum_k b_k + Ns t) / (2 Ns).
+// the sums of the lever arms of each body's sites, s_i = R(q_i) sum_k b_k and
+// s_j = R(q_j) sum_k b_k, and the centroid of the 2*Ns sites,
+//   c = (s_i + s_j + Ns t) / (2 Ns).
 struct PairIn {
     float4 qi, qj;
     float tx, ty, tz;
     float cx, cy, cz;
+    float six, siy, siz, sjx, sjy, sjz;
 };
 
 template <class Cfg>
@@ -162,13 +164,12 @@ __device__ __forceinline__ PairIn pair_in(const nepx_pair_model& m, float4 qi, f
     p.tx = tx;
     p.ty = ty;
     p.tz = tz;
-    float ax, ay, az, bx, by, bz;
-    mat_vec(quat_matrix(qi.x, qi.y, qi.z, qi.w), sx, sy, sz, ax, ay, az);
-    mat_vec(quat_matrix(qj.x, qj.y, qj.z, qj.w), sx, sy, sz, bx, by, bz);
+    mat_vec(quat_matrix(qi.x, qi.y, qi.z, qi.w), sx, sy, sz, p.six, p.siy, p.siz);
+    mat_vec(quat_matrix(qj.x, qj.y, qj.z, qj.w), sx, sy, sz, p.sjx, p.sjy, p.sjz);
     const float inv = 1.f / (float)Cfg::ND;
-    p.cx = (ax + bx) * inv + 0.5f * tx;
-    p.cy = (ay + by) * inv + 0.5f * ty;
-    p.cz = (az + bz) * inv + 0.5f * tz;
+    p.cx = (p.six + p.sjx) * inv + 0.5f * tx;
+    p.cy = (p.siy + p.sjy) * inv + 0.5f * ty;
+    p.cz = (p.siz + p.sjz) * inv + 0.5f * tz;
     return p;
 }
 
@@ -389,15 +390,15 @@ __device__ __forceinline__ void desc_finish(const float (&A)[Cfg::NANG + 1][LR::
 // range LR, c_lm = sum_n B_nlm f_n(rho_a) and d_lm = sum_n B_nlm f_n'(rho_a).
 // Adds sum_lm d_lm Z_lm to dEdrho and returns
 //   (gx, gy, gz) = grad sum_lm c_lm Z_lm  (Z_lm extended to homogeneous polynomials of degree l)
-//   proj         = sum_lm l c_lm Z_lm(u_a),
+//   proj         = sum_lm l c_lm Z_lm(u_a) = u_a . (gx, gy, gz)  (Euler's theorem for the
+//                  homogeneous extension of degree l)
 // using  d/dx Q_l^m = x Q_{l-1}^{m+1},  d/dy Q_l^m = y Q_{l-1}^{m+1},  d/dz Q_l^m = (l+m) Q_{l-1}^m,
 //        d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}),
 // with the sums over l taken first for each m, so each (l, m) costs a few FMAs.
 template <class LR, class CD>
-__device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, CD&& cd,
+__device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, float uz, CD&& cd,
                                                   float& dEdrho, float& gx, float& gy, float& gz, float& proj) {
     float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f;
-    proj = 0.f;
 #pragma unroll
     for (int mm = 0; mm <= LR::L1; ++mm) {
         float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
@@ -406,7 +407,6 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
             const float ql = h.q(l, mm);
             const float qa = (mm + 1 <= l - 1) ? h.q(l - 1, mm + 1) : 0.f;
             const float qc = (mm <= l - 1) ? (float)(l + mm) * h.q(l - 1, mm) : 0.f;
-            float pl = 0.f;
 #pragma unroll
             for (int part = 0; part < (mm ? 2 : 1); ++part) {
                 const int o = lm_slot(l, mm, part) - LR::OFF;
@@ -414,12 +414,10 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
                 float c, d;
                 cd(o, c, d);
                 dEdrho += d * z;
-                pl += c * z;
                 if (mm + 1 <= l - 1) U[part] += qa * c;
                 if (mm <= l - 1) V[part] += qc * c;
                 if (mm >= 1) W[part] += ql * c;
             }
-            proj += (float)l * pl;
         }
         if (mm == 0) {
             E1 += U[0];
@@ -435,6 +433,7 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
     gx = ux * E1 + X1;
     gy = uy * E1 + Y1;
     gz = Gz;
+    proj = ux * gx + uy * gy + uz * gz;
 }
 
 // dE/dr_a = dE/drho_a u_a / r_nep + (grad - proj u_a) / |r_a|.
@@ -453,7 +452,8 @@ struct PairOut {
     float tjx, tjy, tjz;  // torque on j about X_j
 };
 
-// Per-body sums of dE/dr_a, arm x dE/dr_a and arms (body 0 = i, 1 = j), summed over passes.
+// Per-body sums of dE/dr_a, arm x dE/dr_a and arms (body 0 = i, 1 = j), summed over passes;
+// the arm sums are the pair's s_i and s_j (PairIn).
 struct AdjAcc {
     float G0x = 0.f, G0y = 0.f, G0z = 0.f, T0x = 0.f, T0y = 0.f, T0z = 0.f, A0x = 0.f, A0y = 0.f, A0z = 0.f;
     float G1x = 0.f, G1y = 0.f, G1z = 0.f, T1x = 0.f, T1y = 0.f, T1z = 0.f, A1x = 0.f, A1y = 0.f, A1z = 0.f;
@@ -490,7 +490,10 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
     }
     const float inv_rn = 1.f / m.r_nep;
     float gx_s = 0.f, gy_s = 0.f, gz_s = 0.f, tx_s = 0.f, ty_s = 0.f, tz_s = 0.f;
-    float ax_s = 0.f, ay_s = 0.f, az_s = 0.f;
+    if constexpr (RAD) {
+        acc.A0x = p.six; acc.A0y = p.siy; acc.A0z = p.siz;
+        acc.A1x = p.sjx; acc.A1y = p.sjy; acc.A1z = p.sjz;
+    }
     PairSites<Cfg>{m, p, inv_rn}.visit(
         [&](const SiteGeo& s, float) {
             float fa[Cfg::NANG + 1], da[Cfg::NANG + 1];
@@ -505,7 +508,7 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
             Harm<LR::L1> h;
             h.eval(s.ux, s.uy, s.uz);
             float gx, gy, gz, proj;
-            angular_site_grad<LR>(h, s.ux, s.uy,
+            angular_site_grad<LR>(h, s.ux, s.uy, s.uz,
                                   [&](int o, float& c, float& d) {
                                       c = 0.f;
                                       d = 0.f;
@@ -524,23 +527,16 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
             tx_s += s.ay * ga_z - s.az * ga_y;
             ty_s += s.az * ga_x - s.ax * ga_z;
             tz_s += s.ax * ga_y - s.ay * ga_x;
-            if constexpr (RAD) {
-                ax_s += s.ax;
-                ay_s += s.ay;
-                az_s += s.az;
-            }
         },
         [&](int body) {
             if (body == 0) {
                 acc.G0x += gx_s; acc.G0y += gy_s; acc.G0z += gz_s;
                 acc.T0x += tx_s; acc.T0y += ty_s; acc.T0z += tz_s;
-                acc.A0x += ax_s; acc.A0y += ay_s; acc.A0z += az_s;
             } else {
                 acc.G1x += gx_s; acc.G1y += gy_s; acc.G1z += gz_s;
                 acc.T1x += tx_s; acc.T1y += ty_s; acc.T1z += tz_s;
-                acc.A1x += ax_s; acc.A1y += ay_s; acc.A1z += az_s;
             }
-            gx_s = gy_s = gz_s = tx_s = ty_s = tz_s = ax_s = ay_s = az_s = 0.f;
+            gx_s = gy_s = gz_s = tx_s = ty_s = tz_s = 0.f;
         });
 }
 
@@ -560,10 +556,6 @@ __device__ __forceinline__ PairOut adj_finish(const AdjAcc& a) {
     o.tjy = -(a.T1y - (a.A1z * bx - a.A1x * bz));
     o.tjz = -(a.T1z - (a.A1x * by - a.A1y * bx));
     return o;
-}
-
-__device__ __forceinline__ float clipf(float v, float c) {
-    return c > 0.f ? fminf(fmaxf(v, -c), c) : v;
 }
 
 }  // namespace nepx

@@ -42,16 +42,17 @@ struct NetLayout {
 
 __host__ __device__ constexpr int round32(int v) { return (v + 31) / 32 * 32; }
 
-// Offsets (in halves) of the fp16 hi/lo operands of the split-precision path.
-// Each (M tile, 32-deep k-step) holds 64 lanes x 8 halves; the lo part of a
-// matrix follows its hi part.  Input rows are padded to a multiple of 32.
+// Offsets (in 16-bit elements) of the operands of the 16x16x32 paths (fp16 or
+// bf16).  Each (M tile, 32-deep k-step) holds 64 lanes x 8 elements; with two
+// parts (hi/lo split) the lo part of a matrix follows its hi part.  Input rows
+// are padded to a multiple of 32.
 struct NetLayoutH {
-    int DPH, DP, HP, NL;
-    __host__ __device__ constexpr int fwd_size(int k) const {  // hi + lo
-        return 2 * (HP / 16) * ((k == 0 ? DPH : HP) / 32) * 512;
+    int DPH, DP, HP, NL, PARTS;
+    __host__ __device__ constexpr int fwd_size(int k) const {
+        return PARTS * (HP / 16) * ((k == 0 ? DPH : HP) / 32) * 512;
     }
     __host__ __device__ constexpr int bwd_size(int k) const {
-        return 2 * ((k == 0 ? DP : HP) / 16) * (HP / 32) * 512;
+        return PARTS * ((k == 0 ? DP : HP) / 16) * (HP / 32) * 512;
     }
     __host__ __device__ constexpr int fwd_off(int k) const {
         return k == 0 ? 0 : fwd_off(k - 1) + fwd_size(k - 1);
@@ -71,10 +72,9 @@ struct NetCfg {
     static constexpr int MD = DP / 16;   // M tiles of dE/dG
     static constexpr int DPH = round32(DP);
     static constexpr int MDH = DPH / 16; // input chunks of the fp16 path
-    static constexpr bool HALF_OK = (HP % 32) == 0;
     static_assert(MD <= MT, "dE/dG tiles reuse the hidden-layer accumulators");
     static constexpr NetLayout layout() { return NetLayout{DP, HP, NL}; }
-    static constexpr NetLayoutH layout_h() { return NetLayoutH{DPH, DP, HP, NL}; }
+    static constexpr NetLayoutH layout_h(int parts) { return NetLayoutH{DPH, DP, HP, NL, parts}; }
 };
 
 // tanh from its definition, branch-free: 1 - 2/(1 + e^{2z}) (hardware exp2 and
@@ -214,47 +214,100 @@ __device__ __forceinline__ void ffnn_tiles(const float* __restrict__ W, int act,
 }
 
 // ---------------------------------------------------------------------------
-// Split-precision network: every product x*w is formed as
-// x_hi w_hi + x_hi w_lo + x_lo w_hi with fp16 parts (x = x_hi + x_lo exactly
-// to ~22 bits) on v_mfma_f32_16x16x32_f16, FP32 accumulation.  K-step s of 32
-// takes, in lane (p, g), units 16*(2s + j/4) + 4g + j%4 for j = 0..7: the two
-// 16-unit accumulator tiles 2s, 2s+1 of the previous layer, so there is still
-// no cross-lane movement.  Deltas are scaled by 2^10 before the split (exact)
-// to stay clear of fp16 underflow, and the products scaled back.
+// 16-bit operand networks on the 16x16x32 matrix-core instructions, FP32
+// accumulation.  K-step s of 32 takes, in lane (p, g), units
+// 16*(2s + j/4) + 4g + j%4 for j = 0..7: the two 16-unit accumulator tiles 2s,
+// 2s+1 of the previous layer, so there is still no cross-lane movement.
+//  * NEPX_NET_F16X3: every product x*w is x_hi w_hi + x_hi w_lo + x_lo w_hi with
+//    fp16 parts (x = x_hi + x_lo to ~22 bits).  The backward deltas are carried
+//    without the energy range (so their size does not grow with en_max - en_min)
+//    and scaled by 2^10 before the split (exact) to stay clear of fp16
+//    underflow; the range and 2^-10 are applied once to dE/dG.  Valid while
+//    |G|, |h| < 65504 and the range-free deltas stay below 64 (tanh hidden
+//    layers are bounded by 1).
+//  * NEPX_NET_BF16X3: the same three products with bf16 parts (~16 bits); bf16
+//    has the FP32 exponent range, so no scaling and no range limit.
+//  * NEPX_NET_BF16: one bf16 product per k-step (8-bit mantissas, round to
+//    nearest even on both operands), the fastest and least accurate form.
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ void split8(const v4f& a, const v4f& b, float sc, v8h& hi, v8h& lo) {
+template <int PREC>
+struct Lowp;
+template <>
+struct Lowp<NEPX_NET_F16X3> {
+    using T = _Float16;
+    using V = v8h;
+    static constexpr int PARTS = 2;
+    static constexpr float UP = 1024.f;
+    __device__ __forceinline__ static v4f mma(const V& a, const V& b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Lowp<NEPX_NET_BF16X3> {
+    using T = __bf16;
+    using V = v8bf;
+    static constexpr int PARTS = 2;
+    static constexpr float UP = 1.f;
+    __device__ __forceinline__ static v4f mma(const V& a, const V& b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Lowp<NEPX_NET_BF16> {
+    using T = __bf16;
+    using V = v8bf;
+    static constexpr int PARTS = 1;
+    static constexpr float UP = 1.f;
+    __device__ __forceinline__ static v4f mma(const V& a, const V& b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Eight consecutive K inputs (two 4-chunks) as the B operand: hi (and lo) parts.
+template <int PREC>
+__device__ __forceinline__ void split8(const v4f& a, const v4f& b, float sc, typename Lowp<PREC>::V& hi,
+                                       typename Lowp<PREC>::V& lo) {
+    using T = typename Lowp<PREC>::T;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const float x = (j < 4 ? a[j] : b[j - 4]) * sc;
-        const _Float16 h = (_Float16)x;
+        const T h = (T)x;
         hi[j] = h;
-        lo[j] = (_Float16)(x - (float)h);
+        if constexpr (Lowp<PREC>::PARTS == 2) lo[j] = (T)(x - (float)h);
     }
 }
 
-__device__ __forceinline__ v4f mfma3(const v8h* __restrict__ A, int idx, int lo_off, const v8h& bh, const v8h& bl,
-                                     v4f acc) {
-    const v8h ah = A[idx], al = A[idx + lo_off];
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
-    return acc;
+template <int PREC>
+__device__ __forceinline__ v4f mma_split(const typename Lowp<PREC>::V* __restrict__ A, int idx, int lo_off,
+                                         const typename Lowp<PREC>::V& bh, const typename Lowp<PREC>::V& bl,
+                                         v4f acc) {
+    using L = Lowp<PREC>;
+    const typename L::V ah = A[idx];
+    if constexpr (L::PARTS == 2) {
+        const typename L::V al = A[idx + lo_off];
+        acc = L::mma(al, bh, acc);
+        acc = L::mma(ah, bl, acc);
+    }
+    return L::mma(ah, bh, acc);
 }
 
 // Same contract as ffnn_tiles; gin has MDH chunks (zero past D).  W32 supplies
-// biases and the output layer, WH the fp16 operands (global or LDS).
-template <class N, int NT>
-__device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, const _Float16* __restrict__ WH, int act,
+// biases and the output layer, WH the 16-bit operands (global or LDS).
+template <class N, int NT, int PREC>
+__device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, const void* __restrict__ WH, int act,
                                              const v4f (&gin)[NT][N::MDH], float scale, v4f (&dgo)[NT][N::MD],
                                              float (&y)[NT]) {
+    using L = Lowp<PREC>;
+    using V = typename L::V;
     constexpr NetLayout Lay = N::layout();
-    constexpr NetLayoutH LH = N::layout_h();
-    constexpr float kUp = 1024.f, kDown = 1.f / 1024.f;
+    constexpr NetLayoutH LH = N::layout_h(L::PARTS);
+    constexpr float kUp = L::UP, kDown = 1.f / L::UP;
     int lane = threadIdx.x & 63;
     asm volatile("" : "+v"(lane));
     const int g = lane >> 4;
-    const v8h* A8 = reinterpret_cast<const v8h*>(WH);
+    const V* A8 = reinterpret_cast<const V*>(WH);
     v4f h[N::NL][NT][N::MT];
 #pragma unroll
     for (int k = 0; k < N::NL; ++k) {
@@ -271,17 +324,17 @@ __device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, cons
         }
 #pragma unroll
         for (int s = 0; s < ks; ++s) {
-            v8h bh[NT], bl[NT];
+            V bh[NT], bl[NT];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                if (k == 0) split8(gin[t][2 * s], gin[t][2 * s + 1], 1.f, bh[t], bl[t]);
-                else split8(h[k - 1][t][2 * s], h[k - 1][t][2 * s + 1], 1.f, bh[t], bl[t]);
+                if (k == 0) split8<PREC>(gin[t][2 * s], gin[t][2 * s + 1], 1.f, bh[t], bl[t]);
+                else split8<PREC>(h[k - 1][t][2 * s], h[k - 1][t][2 * s + 1], 1.f, bh[t], bl[t]);
             }
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) {
                 const int idx = base + (mt * ks + s) * 64 + lane;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t][mt] = mfma3(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
+                for (int t = 0; t < NT; ++t) acc[t][mt] = mma_split<PREC>(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
             }
         }
 #pragma unroll
@@ -304,7 +357,7 @@ __device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, cons
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 y[t] += wv[r] * h[N::NL - 1][t][mt][r];
-                delta[t][mt][r] = scale * wv[r] * act_der(h[N::NL - 1][t][mt][r], act);
+                delta[t][mt][r] = wv[r] * act_der(h[N::NL - 1][t][mt][r], act);
             }
     }
     const float bo = W32[Lay.bout_off()];
@@ -327,15 +380,15 @@ __device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, cons
             for (int mt = 0; mt < N::MT; ++mt) acc[t][mt] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < ks; ++s) {
-            v8h bh[NT], bl[NT];
+            V bh[NT], bl[NT];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) split8(delta[t][2 * s], delta[t][2 * s + 1], kUp, bh[t], bl[t]);
+            for (int t = 0; t < NT; ++t) split8<PREC>(delta[t][2 * s], delta[t][2 * s + 1], kUp, bh[t], bl[t]);
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) {
                 if (mt >= mtn) break;
                 const int idx = base + (mt * ks + s) * 64 + lane;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t][mt] = mfma3(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
+                for (int t = 0; t < NT; ++t) acc[t][mt] = mma_split<PREC>(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
             }
         }
         if (k > 0) {
@@ -350,7 +403,7 @@ __device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, cons
 #pragma unroll
             for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int mt = 0; mt < N::MD; ++mt) dgo[t][mt] = acc[t][mt] * kDown;
+                for (int mt = 0; mt < N::MD; ++mt) dgo[t][mt] = acc[t][mt] * (kDown * scale);
         }
     }
 }

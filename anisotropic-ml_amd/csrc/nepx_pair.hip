@@ -85,16 +85,16 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_mode
     }
 }
 
-template <class N, int NT, bool HALF, int ACT>
+template <class N, int NT, int PREC, int ACT>
 __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, const void* __restrict__ Whg,
                                                  float en_min, float scale, int D, const float* __restrict__ G,
                                                  const int32_t* __restrict__ np_dev, int64_t maxp,
                                                  float* __restrict__ E, float* __restrict__ dEdG) {
-    using R = NetRun<N, NT, HALF>;
-    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, HALF>()];
+    using R = NetRun<N, NT, PREC>;
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, PREC>()];
     const float* W;
-    const _Float16* WH;
-    stage_net<N, HALF>(wl, Wg, Whg, W, WH);
+    const void* WH;
+    stage_net<N, PREC>(wl, Wg, Whg, W, WH);
     const int P = n_pairs(np_dev, maxp);
     const int lane = threadIdx.x & 63;
     const int nt = (P + 16 * NT - 1) / (16 * NT);
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, c
 // block runs the same number of iterations and the barriers are uniform.
 // The forward accumulators A of each lane's pair stay in registers through
 // the network phase and feed the adjoint directly.
-template <class Cfg, class N, bool HALF, int ACT>
+template <class Cfg, class N, int PREC, int ACT>
 __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model m, const float* __restrict__ Wg,
                                                                 const void* __restrict__ Whg,
                                                                 float en_min, float scale,
@@ -137,11 +137,11 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
     constexpr int LD = (Cfg::D + 3) / 4 * 4 + 4;  // row stride in LDS (16-byte aligned)
     __shared__ __attribute__((aligned(16))) float tile[kBlock * LD];
     __shared__ float etile[kBlock];
-    using R = NetRun<N, kNetTiles, HALF>;
-    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, HALF>()];
+    using R = NetRun<N, kNetTiles, PREC>;
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, PREC>()];
     const float* W;
-    const _Float16* WH;
-    stage_net<N, HALF>(wl, Wg, Whg, W, WH);
+    const void* WH;
+    stage_net<N, PREC>(wl, Wg, Whg, W, WH);
     const int P = n_pairs(np_dev, maxp);
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -385,25 +385,55 @@ extern "C" int nepx_net_pack(int32_t d_in, int32_t width, int32_t n_hidden, cons
     return NEPX_OK;
 }
 
-extern "C" int64_t nepx_net_packed_halves(int32_t d_in, int32_t width, int32_t n_hidden) {
-    if (d_in <= 0 || width <= 0 || n_hidden < 0 || round16(width) % 32) return -1;
-    NetLayoutH L{round32(round16(d_in)), round16(d_in), round16(width), n_hidden + 1};
+static bool lowp_precision(int p) {
+    return p == NEPX_NET_F16X3 || p == NEPX_NET_BF16 || p == NEPX_NET_BF16X3;
+}
+
+// float -> bf16 bits, round to nearest even (NaN stays NaN).
+static uint16_t bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+extern "C" int64_t nepx_net_packed_lowp(int32_t precision, int32_t d_in, int32_t width, int32_t n_hidden) {
+    if (!lowp_precision(precision) || d_in <= 0 || width <= 0 || n_hidden < 0 || round16(width) % 32) return -1;
+    NetLayoutH L{round32(round16(d_in)), round16(d_in), round16(width), n_hidden + 1,
+                 precision == NEPX_NET_BF16 ? 1 : 2};
     return L.total();
 }
 
-extern "C" int nepx_net_pack_half(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
-                                  uint16_t* out) {
+extern "C" int nepx_net_pack_lowp(int32_t precision, int32_t d_in, int32_t width, int32_t n_hidden,
+                                  const float* const* ws, uint16_t* out) {
+    NEPX_REQUIRE(lowp_precision(precision), NEPX_EINVAL, "precision %d has no 16-bit operands", precision);
     NEPX_REQUIRE(d_in > 0 && width > 0 && n_hidden >= 0, NEPX_EINVAL, "bad network dims");
-    NEPX_REQUIRE(round16(width) % 32 == 0, NEPX_EUNSUP, "fp16 path needs a padded width that is a multiple of 32");
+    NEPX_REQUIRE(round16(width) % 32 == 0, NEPX_EUNSUP,
+                 "16-bit operand paths need a padded width that is a multiple of 32");
     NEPX_REQUIRE(ws && out, NEPX_EINVAL, "null pointer");
+    const int parts = precision == NEPX_NET_BF16 ? 1 : 2;
     const int DP = round16(d_in), DPH = round32(DP), HP = round16(width), NL = n_hidden + 1;
-    NetLayoutH L{DPH, DP, HP, NL};
-    _Float16* o = reinterpret_cast<_Float16*>(out);
-    for (int64_t k = 0; k < L.total(); ++k) o[k] = (_Float16)0.f;
+    NetLayoutH L{DPH, DP, HP, NL, parts};
+    for (int64_t k = 0; k < L.total(); ++k) out[k] = 0;
+    // hi = w rounded to the 16-bit type, lo = the remainder rounded (split forms only)
     auto put = [&](int base, int lo_off, int idx, float w) {
-        const _Float16 h = (_Float16)w;
-        o[base + idx] = h;
-        o[base + lo_off + idx] = (_Float16)(w - (float)h);
+        if (precision == NEPX_NET_F16X3) {
+            const _Float16 h = (_Float16)w;
+            const _Float16 l = (_Float16)(w - (float)h);
+            memcpy(&out[base + idx], &h, 2);
+            memcpy(&out[base + lo_off + idx], &l, 2);
+        } else {
+            const uint16_t h = bf16_bits(w);
+            out[base + idx] = h;
+            if (parts == 2) out[base + lo_off + idx] = bf16_bits(w - bf16_value(h));
+        }
     };
     for (int k = 0; k < NL; ++k) {
         const int kin = k == 0 ? d_in : width;
@@ -433,6 +463,15 @@ extern "C" int nepx_net_pack_half(int32_t d_in, int32_t width, int32_t n_hidden,
                     }
     }
     return NEPX_OK;
+}
+
+extern "C" int64_t nepx_net_packed_halves(int32_t d_in, int32_t width, int32_t n_hidden) {
+    return nepx_net_packed_lowp(NEPX_NET_F16X3, d_in, width, n_hidden);
+}
+
+extern "C" int nepx_net_pack_half(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
+                                  uint16_t* out) {
+    return nepx_net_pack_lowp(NEPX_NET_F16X3, d_in, width, n_hidden, ws, out);
 }
 
 extern "C" int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
@@ -468,24 +507,49 @@ static bool net_supported(const nepx_net* n) {
     return false;
 }
 
-template <class T, int NT, bool HALF_OK>
-static int launch_ffnn(const nepx_net* net, bool half, int grid, float scale, int d, const float* G,
-                       const int32_t* np_dev, int64_t maxp, float* E, float* dEdG, void* stream) {
-    const bool tanh_act = net->act == NEPX_ACT_TANH;
-#define NEPX_LAUNCH_FFNN(H, A)                                                                             \
-    hipLaunchKernelGGL((k_ffnn<T, NT, H, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), net->w_packed, \
+static int check_precision(const nepx_net* net) {
+    const int p = net->precision;
+    NEPX_REQUIRE(p == NEPX_NET_F32 || lowp_precision(p), NEPX_EINVAL, "bad network precision %d", p);
+    NEPX_REQUIRE(p == NEPX_NET_F32 || net->w_half, NEPX_EINVAL, "precision %d without packed 16-bit operands", p);
+    NEPX_REQUIRE(p == NEPX_NET_F32 || round16(net->width) % 32 == 0, NEPX_EUNSUP,
+                 "no 16-bit network kernel for width %d", net->width);
+    // bf16 rounding moves pre-activations across zero, where ReLU's derivative jumps
+    NEPX_REQUIRE((p != NEPX_NET_BF16 && p != NEPX_NET_BF16X3) || net->act == NEPX_ACT_TANH, NEPX_EUNSUP,
+                 "the bf16 network forms are for tanh networks only");
+    return NEPX_OK;
+}
+
+template <class T, int NT, int PREC>
+static void launch_ffnn_p(const nepx_net* net, int grid, float scale, int d, const float* G, const int32_t* np_dev,
+                          int64_t maxp, float* E, float* dEdG, void* stream) {
+#define NEPX_LAUNCH_FFNN(A)                                                                                   \
+    hipLaunchKernelGGL((k_ffnn<T, NT, PREC, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), net->w_packed, \
                        net->w_half, net->en_min, scale, d, G, np_dev, maxp, E, dEdG)
-    if constexpr (HALF_OK) {
-        if (half) {
-            if (tanh_act) NEPX_LAUNCH_FFNN(true, NEPX_ACT_TANH);
-            else NEPX_LAUNCH_FFNN(true, NEPX_ACT_RELU);
-            NEPX_LAUNCH_CHECK();
-            return NEPX_OK;
-        }
-    }
-    if (tanh_act) NEPX_LAUNCH_FFNN(false, NEPX_ACT_TANH);
-    else NEPX_LAUNCH_FFNN(false, NEPX_ACT_RELU);
+    if (net->act == NEPX_ACT_TANH) NEPX_LAUNCH_FFNN(NEPX_ACT_TANH);
+    else if constexpr (PREC == NEPX_NET_F32 || PREC == NEPX_NET_F16X3) NEPX_LAUNCH_FFNN(NEPX_ACT_RELU);
 #undef NEPX_LAUNCH_FFNN
+}
+
+template <class T, int NT>
+static int launch_ffnn(const nepx_net* net, int grid, float scale, int d, const float* G, const int32_t* np_dev,
+                       int64_t maxp, float* E, float* dEdG, void* stream) {
+    if constexpr (T::HP % 32 == 0) {
+        switch (net->precision) {
+            case NEPX_NET_F16X3:
+                launch_ffnn_p<T, NT, NEPX_NET_F16X3>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);
+                break;
+            case NEPX_NET_BF16:
+                launch_ffnn_p<T, NT, NEPX_NET_BF16>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);
+                break;
+            case NEPX_NET_BF16X3:
+                launch_ffnn_p<T, NT, NEPX_NET_BF16X3>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);
+                break;
+            default:
+                launch_ffnn_p<T, NT, NEPX_NET_F32>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);
+        }
+    } else {
+        launch_ffnn_p<T, NT, NEPX_NET_F32>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);
+    }
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
@@ -496,16 +560,15 @@ extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int3
     NEPX_REQUIRE(G && np_dev && E && dEdG, NEPX_EINVAL, "null buffer");
     NEPX_REQUIRE(net_supported(net), NEPX_EUNSUP, "no network kernel for %d -> %d x%d", net->d_in,
                  net->width, net->n_hidden + 1);
+    int rc = check_precision(net);
+    if (rc) return rc;
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-    const bool half = net->precision == NEPX_NET_F16X3;
-    NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
 #define X(d, w, h, T, NT)                                                                        \
     if (net->d_in == d && net->width == w && net->n_hidden == h) {                               \
         const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64,            \
                                       (int64_t)device_cus() * tuning().net_blocks_per_cu);       \
-        if constexpr (!T::HALF_OK) NEPX_REQUIRE(!half, NEPX_EUNSUP, "no fp16 network kernel for width %d", w); \
-        return launch_ffnn<T, NT, T::HALF_OK>(net, half, grid, scale, d, G, np_dev, maxp, E, dEdG, stream); \
+        return launch_ffnn<T, NT>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);       \
     }
     NEPX_NET_CASES(X)
 #undef X
@@ -539,42 +602,64 @@ extern "C" int nepx_descriptor_adjoint(const nepx_pair_model* m, const float* qf
     X(10, 11, 6, 4, CfgTet10, 40, 64, 1, Net40x64) \
     X(12, 17, 6, 6, CfgCyl12, 60, 128, 1, Net60x128)
 
+// The bf16 forms exist for tanh networks only (check_precision).
+static bool fused_precision_ok(const nepx_net* net) {
+    return net->precision == NEPX_NET_F32 || net->precision == NEPX_NET_F16X3 || net->act == NEPX_ACT_TANH;
+}
+
 extern "C" int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_net* net) {
     if (!m || !net) return 0;
 #define X(ns, a, b, c, T, d, w, h, NT)                                                            \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&    \
-        net->width == w && net->n_hidden == h)                                                   \
+        net->width == w && net->n_hidden == h && fused_precision_ok(net))                        \
         return QuadCfg<T>::OK && tuning().pair_lanes == 4 ? 4 : 1;
     NEPX_FUSED_CASES(X)
 #undef X
     return 0;
 }
 
+template <class Cfg, class N, int PREC>
+static void launch_fused_p(const nepx_pair_model& m, const nepx_net* net, int grid, float scale, const float* qf,
+                           const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                           const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+#define NEPX_LAUNCH_FUSED(A)                                                                                     \
+    hipLaunchKernelGGL((k_fused<Cfg, N, PREC, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), m,            \
+                       net->w_packed, net->w_half, net->en_min, scale, (const float4*)qf, pair_i, pair_j,          \
+                       (const float4*)pair_t, np_dev, maxp, clip, out)
+    if (net->act == NEPX_ACT_TANH) NEPX_LAUNCH_FUSED(NEPX_ACT_TANH);
+    else if constexpr (PREC == NEPX_NET_F32 || PREC == NEPX_NET_F16X3) NEPX_LAUNCH_FUSED(NEPX_ACT_RELU);
+#undef NEPX_LAUNCH_FUSED
+}
+
 template <class Cfg, class N>
-static int launch_fused(const nepx_pair_model& m, const nepx_net* net, bool half, float scale,
+static int launch_fused(const nepx_pair_model& m, const nepx_net* net, float scale,
                         const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
                         const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
     const int64_t cap = (int64_t)device_cus() * tuning().fused_blocks_per_cu;
     // four lanes per pair (site sums on MFMA) where the shape allows it, unless tuned off
     if (tuning().pair_lanes == 4) {
-        const int rc = launch_fused_quad<Cfg, N>(m, net, half, (int)cap, scale, qf, pair_i, pair_j, pair_t,
-                                                 np_dev, maxp, clip, out, stream);
+        const int rc = launch_fused_quad<Cfg, N>(m, net, net->precision, (int)cap, scale, qf, pair_i, pair_j,
+                                                 pair_t, np_dev, maxp, clip, out, stream);
         if (rc != NEPX_EUNSUP) return rc;
     }
     const int grid = grid_for_cap(maxp, kBlock, cap);
-    const bool tanh_act = net->act == NEPX_ACT_TANH;
-#define NEPX_LAUNCH_FUSED(H, A)                                                                                  \
-    hipLaunchKernelGGL((k_fused<Cfg, N, H, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), m, net->w_packed, \
-                       net->w_half, net->en_min, scale, (const float4*)qf, pair_i, pair_j, (const float4*)pair_t,  \
-                       np_dev, maxp, clip, out)
-    if (half) {
-        if (tanh_act) NEPX_LAUNCH_FUSED(true, NEPX_ACT_TANH);
-        else NEPX_LAUNCH_FUSED(true, NEPX_ACT_RELU);
-    } else {
-        if (tanh_act) NEPX_LAUNCH_FUSED(false, NEPX_ACT_TANH);
-        else NEPX_LAUNCH_FUSED(false, NEPX_ACT_RELU);
+    switch (net->precision) {
+        case NEPX_NET_F16X3:
+            launch_fused_p<Cfg, N, NEPX_NET_F16X3>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp,
+                                                    clip, out, stream);
+            break;
+        case NEPX_NET_BF16:
+            launch_fused_p<Cfg, N, NEPX_NET_BF16>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp,
+                                                   clip, out, stream);
+            break;
+        case NEPX_NET_BF16X3:
+            launch_fused_p<Cfg, N, NEPX_NET_BF16X3>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev,
+                                                     maxp, clip, out, stream);
+            break;
+        default:
+            launch_fused_p<Cfg, N, NEPX_NET_F32>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp,
+                                                  clip, out, stream);
     }
-#undef NEPX_LAUNCH_FUSED
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
@@ -586,16 +671,15 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
     if (rc) return rc;
     rc = check_net(net, model_D(m));
     if (rc) return rc;
+    rc = check_precision(net);
+    if (rc) return rc;
     NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && out, NEPX_EINVAL, "null buffer");
     if (maxp <= 0) return NEPX_OK;
     const float scale = net->en_max - net->en_min;
-    const bool half = net->precision == NEPX_NET_F16X3;
-    NEPX_REQUIRE(!half || net->w_half, NEPX_EINVAL, "fp16 precision without packed fp16 operands");
 #define X(ns, a, b, c, T, d, w, h, NT)                                                               \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&        \
         net->width == w && net->n_hidden == h)                                                       \
-        return launch_fused<T, NT>(*m, net, half, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip,       \
-                                   out, stream);
+        return launch_fused<T, NT>(*m, net, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip, out, stream);
     NEPX_FUSED_CASES(X)
 #undef X
     set_error("no fused kernel for n_sites=%d hypers=(%d,%d,%d) net %d->%dx%d", m->n_sites, m->nrad,

@@ -98,44 +98,48 @@ __device__ __forceinline__ void scatter_chunks(float* __restrict__ row, bool val
 
 // Networks whose packed operands fit this budget are staged in LDS once per block.
 constexpr int kWeightLdsBytes = 64 * 1024;
-template <class N, bool HALF>
+template <int PREC>
+constexpr int lowp_parts() {
+    return PREC == NEPX_NET_BF16 ? 1 : 2;
+}
+template <class N, int PREC>
 constexpr int operand_floats() {
-    return HALF ? N::layout_h().total() / 2 : N::layout().total();
+    return PREC == NEPX_NET_F32 ? N::layout().total() : N::layout_h(lowp_parts<PREC>()).total() / 2;
 }
-template <class N, bool HALF>
+template <class N, int PREC>
 constexpr bool weights_in_lds() {
-    return operand_floats<N, HALF>() * 4 <= kWeightLdsBytes;
+    return operand_floats<N, PREC>() * 4 <= kWeightLdsBytes;
 }
-template <class N, bool HALF>
+template <class N, int PREC>
 constexpr int weight_lds_floats() {
-    return weights_in_lds<N, HALF>() ? operand_floats<N, HALF>() : 4;
+    return weights_in_lds<N, PREC>() ? operand_floats<N, PREC>() : 4;
 }
 
 constexpr int kNetTiles = 2;  // 16-pair tiles sharing each A-operand read
 
 // Network over NT tiles of 16 pairs with the operands chosen at compile time.
-template <class N, int NT, bool HALF>
+template <class N, int NT, int PREC>
 struct NetRun {
-    static constexpr int IC = HALF ? N::MDH : N::MD;  // input chunks
-    __device__ __forceinline__ static void run(const float* W, const _Float16* WH, int act, const v4f (&gin)[NT][IC],
+    static constexpr int IC = PREC == NEPX_NET_F32 ? N::MD : N::MDH;  // input chunks
+    __device__ __forceinline__ static void run(const float* W, const void* WH, int act, const v4f (&gin)[NT][IC],
                                                float scale, v4f (&dg)[NT][N::MD], float (&y)[NT]) {
-        if constexpr (HALF) ffnn_tiles_h<N, NT>(W, WH, act, gin, scale, dg, y);
-        else ffnn_tiles<N, NT>(W, act, gin, scale, dg, y);
+        if constexpr (PREC == NEPX_NET_F32) ffnn_tiles<N, NT>(W, act, gin, scale, dg, y);
+        else ffnn_tiles_h<N, NT, PREC>(W, WH, act, gin, scale, dg, y);
     }
 };
 
 // Stage the operands of this precision into LDS when they fit; returns the pointers to use.
-template <class N, bool HALF>
+template <class N, int PREC>
 __device__ __forceinline__ void stage_net(float* wl, const float* Wg, const void* Whg, const float*& W,
-                                          const _Float16*& WH) {
+                                          const void*& WH) {
     W = Wg;
-    WH = reinterpret_cast<const _Float16*>(Whg);
-    if constexpr (weights_in_lds<N, HALF>()) {
-        if constexpr (HALF) {
-            stage_copy(wl, reinterpret_cast<const float*>(Whg), operand_floats<N, HALF>());
-            WH = reinterpret_cast<const _Float16*>(wl);
+    WH = Whg;
+    if constexpr (weights_in_lds<N, PREC>()) {
+        if constexpr (PREC != NEPX_NET_F32) {
+            stage_copy(wl, reinterpret_cast<const float*>(Whg), operand_floats<N, PREC>());
+            WH = wl;
         } else {
-            stage_copy(wl, Wg, operand_floats<N, HALF>());
+            stage_copy(wl, Wg, operand_floats<N, PREC>());
             W = wl;
         }
         __syncthreads();
@@ -179,7 +183,7 @@ struct QuadCfg {
 // sums on matrix cores (nepx_quad.hip), on at most max_blocks blocks.  Returns
 // NEPX_EUNSUP for shapes it does not cover.
 template <class Cfg, class N>
-int launch_fused_quad(const nepx_pair_model& m, const nepx_net* net, bool half, int max_blocks, float scale,
+int launch_fused_quad(const nepx_pair_model& m, const nepx_net* net, int prec, int max_blocks, float scale,
                       const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
                       const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream);
 

@@ -72,7 +72,7 @@ __device__ __forceinline__ float sel4(int q, float v0, float v1, float v2, float
 
 // Each block takes BLOCK/4 consecutive pairs per iteration (16 per wave);
 // waves share nothing but the network weights, so the loop has no block barriers.
-template <class Cfg, class N, bool HALF, int ACT>
+template <class Cfg, class N, int PREC, int ACT>
 __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     amdgpu_waves_per_eu(QuadCfg<Cfg>::WAVES_PER_SIMD, QuadCfg<Cfg>::WAVES_PER_SIMD))) void k_fused_quad(
     nepx_pair_model m, const float* __restrict__ Wg, const void* __restrict__ Whg, float en_min, float scale,
@@ -89,8 +89,8 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     __shared__ __attribute__((aligned(16))) float xch[Q::BLOCK / 64][PW * Q::TS];
     __shared__ float etile[Q::BLOCK / 4];
     __shared__ float4 stab[Cfg::ND];
-    using R = NetRun<N, 1, HALF>;
-    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, HALF>()];
+    using R = NetRun<N, 1, PREC>;
+    __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, PREC>()];
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int a = 0; a < Cfg::ND; ++a) {
@@ -99,8 +99,8 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
         }
     }
     const float* W;
-    const _Float16* WH;
-    stage_net<N, HALF>(wl, Wg, Whg, W, WH);
+    const void* WH;
+    stage_net<N, PREC>(wl, Wg, Whg, W, WH);
     __syncthreads();
 
     const int P = n_pairs(np_dev, maxp);
@@ -259,11 +259,9 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
                 Af[n / 4][d][n % 4] *= 2.f * w;
             }
         }
-        // per-lane sums over its sites on body i and on body j
-        float ig_x = 0.f, ig_y = 0.f, ig_z = 0.f, it_x = 0.f, it_y = 0.f, it_z = 0.f, ia_x = 0.f, ia_y = 0.f,
-              ia_z = 0.f;
-        float jg_x = 0.f, jg_y = 0.f, jg_z = 0.f, jt_x = 0.f, jt_y = 0.f, jt_z = 0.f, ja_x = 0.f, ja_y = 0.f,
-              ja_z = 0.f;
+        // per-lane sums over its sites on body i and on body j (the arm sums are the pair's s_i, s_j)
+        float ig_x = 0.f, ig_y = 0.f, ig_z = 0.f, it_x = 0.f, it_y = 0.f, it_z = 0.f;
+        float jg_x = 0.f, jg_y = 0.f, jg_z = 0.f, jt_x = 0.f, jt_y = 0.f, jt_z = 0.f;
 #pragma unroll 1
         for (int s = 0; s < Q::STEPS; ++s) {
             bool bj;
@@ -295,7 +293,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             h.eval(sg.ux, sg.uy, sg.uz);
             float gx, gy, gz, proj;
             angular_site_grad<LRange<1, LMAX>>(
-                h, sg.ux, sg.uy,
+                h, sg.ux, sg.uy, sg.uz,
                 [&](int o, float& c, float& dv) {
                     c = cc[o / 4][o % 4];
                     dv = dd[o / 4][o % 4];
@@ -307,18 +305,16 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             const float wj = bj ? 1.f : 0.f, wi = 1.f - wj;
             ig_x += wi * ex; ig_y += wi * ey; ig_z += wi * ez;
             it_x += wi * tx; it_y += wi * ty; it_z += wi * tz;
-            ia_x += wi * sg.ax; ia_y += wi * sg.ay; ia_z += wi * sg.az;
             jg_x += wj * ex; jg_y += wj * ey; jg_z += wj * ez;
             jt_x += wj * tx; jt_y += wj * ty; jt_z += wj * tz;
-            ja_x += wj * sg.ax; ja_y += wj * sg.ay; ja_z += wj * sg.az;
         }
         AdjAcc acc;
         acc.G1x = quad_sum(jg_x); acc.G1y = quad_sum(jg_y); acc.G1z = quad_sum(jg_z);
         acc.T1x = quad_sum(jt_x); acc.T1y = quad_sum(jt_y); acc.T1z = quad_sum(jt_z);
-        acc.A1x = quad_sum(ja_x); acc.A1y = quad_sum(ja_y); acc.A1z = quad_sum(ja_z);
+        acc.A1x = g.sjx; acc.A1y = g.sjy; acc.A1z = g.sjz;
         acc.G0x = quad_sum(ig_x); acc.G0y = quad_sum(ig_y); acc.G0z = quad_sum(ig_z);
         acc.T0x = quad_sum(it_x); acc.T0y = quad_sum(it_y); acc.T0z = quad_sum(it_z);
-        acc.A0x = quad_sum(ia_x); acc.A0y = quad_sum(ia_y); acc.A0z = quad_sum(ia_z);
+        acc.A0x = g.six; acc.A0y = g.siy; acc.A0z = g.siz;
         if (valid && q == 0) store_out(out, idx, adj_finish<Cfg>(acc), E, clip);
         wave_lds_order();  // the row reads are done before the next iteration's exchange writes
     }

@@ -37,6 +37,7 @@ class DescriptorGenerator:
         self.fd_compat = False
         self.device = torch.device("cuda:0")
         self._ctx = None
+        self._nlist_obj = None
         self._nlist_key = None
         self.dx = 1e-5
         self.dteta = 1e-5
@@ -98,6 +99,7 @@ class DescriptorGenerator:
                               skin=0.0)
             self._ctx = Arena([spec], self.shape, (self.nrad, self.nang, self.lmax), None, device=self.device,
                               list_cap=4096)
+            self._nlist_obj = None
             self._nlist_key = None
         return self._ctx
 
@@ -128,11 +130,15 @@ class DescriptorGenerator:
         quat = torch.as_tensor(orientations)
         ctx = self._context(pos.shape[0])
         ctx.set_configuration(pos, quat)
-        nl = torch.as_tensor(Nlist)
-        key = (nl.data_ptr(), tuple(nl.shape), getattr(nl, "_version", 0))
-        if key != self._nlist_key:
-            ctx.load_pairs(nl)
-            self._nlist_key = key
+        # Reload unless this is the very tensor loaded last time, unmodified (torch counts in-place
+        # writes in _version).  Holding it keeps its storage from being reused for another list;
+        # arrays of other libraries are converted afresh on every call and always reloaded.
+        same = (isinstance(Nlist, torch.Tensor) and Nlist is self._nlist_obj
+                and Nlist._version == self._nlist_key)
+        if not same:
+            ctx.load_pairs(torch.as_tensor(Nlist))
+            self._nlist_obj = Nlist
+            self._nlist_key = Nlist._version if isinstance(Nlist, torch.Tensor) else None
         ctx.filter_pairs()
         self.N_pair = ctx.n_pairs()   # host read of the pair count, as len(self.pairs) in the reference
         P = self.N_pair
@@ -167,6 +173,3 @@ class DescriptorGenerator:
                                                      self.pair_j.contiguous(), self.pair_t.contiguous())
         return self.g_all_cupy
 
-
-def dummy():
-    pass

@@ -138,6 +138,3 @@ class ForceEvaluator:
         self.forces = F
         self.torks = T
 
-
-def dummy():
-    pass

@@ -33,6 +33,3 @@ class NVT(Sim):
             self.trans_kin_en = float(th["Kt"])
             self.rot_kin_en = float(th["Kr"])
 
-
-def dummy():
-    pass

@@ -222,6 +222,7 @@ class Sim:
         self.arena.build_list(True, self.runner.stream)
 
     def set_timers(self):
+        self.t_run = 0.0
         self.t_gen = 0.0
         self.t_eval = 0.0
         self.t_s1 = 0.0
@@ -239,19 +240,33 @@ class Sim:
         self.epoch_timer = time.time()
         check_every = 100
         base = self.arena.device_steps()
+        dumping = bool(self.gsdDumpPeriod and self.gsdFilename)
+        last_dump = -1
         while self.timestep < Nsteps:
-            if self.gsdDumpPeriod and self.gsdFilename and self.timestep % self.gsdDumpPeriod == 0:
+            if dumping and self.timestep % self.gsdDumpPeriod == 0 and self.timestep != last_dump:
+                # A list overflow freezes the state at its last good step while the host keeps
+                # counting; settle it before writing so every frame's label is the device's count.
                 torch.cuda.synchronize(self.device)
-                self.print_performance_info()
-                self.dumpConfig(self.timestep)
+                self._recover(base)
+                if self.timestep % self.gsdDumpPeriod == 0 and self.timestep != last_dump:
+                    self.print_performance_info()
+                    self.dumpConfig(self.timestep)
+                    last_dump = self.timestep
+                continue
             self.step()
             if self.timestep % check_every == 0 or self.timestep == Nsteps:
                 self._recover(base)
         torch.cuda.synchronize(self.device)
         self.arena.check()
+        self.t_run = time.time() - self.timer
         print("Timers:")
-        for k in ("t_nlist", "t_gen", "t_eval", "t_s1", "t_s2"):
-            print("%s : %.2f" % (k, getattr(self, k)))
+        if self.fd_compat or self.component_path:
+            for k in ("t_nlist", "t_gen", "t_eval", "t_s1", "t_s2"):
+                print("%s : %.2f" % (k, getattr(self, k)))
+        else:
+            # the fast path is one device-only captured step: list test, filter, pair kernel, gather
+            # and NVT are not separated by host synchronisations, so only the whole run is timed
+            print("t_run : %.2f (%d steps, captured device-only steps)" % (self.t_run, Nsteps))
         print("Done")
 
     def _recover(self, base):

@@ -12,10 +12,14 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("NEPX_LIB", os.path.join(_HERE, "libnepx.so"))
 
 MAX_SITES = 16
+OK, EINVAL, EUNSUP, EHIP, ECAP = 0, -1, -2, -3, -4   # return codes of the C ABI
 ACT_TANH = 0
 ACT_RELU = 1
 NET_F32 = 0
 NET_F16X3 = 1
+NET_BF16 = 2
+NET_BF16X3 = 3
+NET_PRECISIONS = {"f32": NET_F32, "f16x3": NET_F16X3, "bf16": NET_BF16, "bf16x3": NET_BF16X3}
 
 c_i32p = C.POINTER(C.c_int32)
 c_f32p = C.POINTER(C.c_float)
@@ -92,6 +96,8 @@ def _load():
                           C.c_int),
         "nepx_net_packed_halves": ([C.c_int32, C.c_int32, C.c_int32], C.c_int64),
         "nepx_net_pack_half": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(c_f32p), vp], C.c_int),
+        "nepx_net_packed_lowp": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32], C.c_int64),
+        "nepx_net_pack_lowp": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(c_f32p), vp], C.c_int),
         "nepx_descriptor_fwd": ([C.POINTER(PairModel), vp, vp, vp, vp, vp, C.c_int64, vp, vp], C.c_int),
         "nepx_ffnn_fwd_bwd": ([C.POINTER(Net), vp, vp, C.c_int64, vp, vp, vp], C.c_int),
         "nepx_descriptor_adjoint": ([C.POINTER(PairModel), vp, vp, vp, vp, vp, C.c_int64, vp, vp, C.c_float,

@@ -1,13 +1,14 @@
-"""Step runner: a dedicated HIP stream, optional hipGraph capture, kernel timing events.
+"""Step runner: a dedicated HIP stream, hipGraph capture, optional kernel timing events.
 
-The step is split into three device-only segments so that the dominant
-kernel can be timed with events on the stream it runs on:
-  pre   = conditional list rebuild + COM filter/compaction
-  pair  = fused descriptor -> MFMA network -> adjoint kernel
-  post  = deterministic gather + FP64 NVT update + thermostat/rebuild test
-With use_graph each segment is captured once into a hipGraph (through
-torch.cuda.CUDAGraph, which drives hipStreamBeginCapture) and replayed; no
-segment contains hipMemsetAsync (all clears are fill kernels).
+A production step is one captured graph replayed per step:
+  conditional list rebuild + COM filter/compaction
+  -> fused descriptor / MFMA network / adjoint kernel
+  -> deterministic gather + FP64 NVT update + thermostat/rebuild test.
+For timing the dominant kernel with events on the stream it runs on, the step
+is recaptured as three graphs (pre / pair / post) while events are enabled;
+the kernels and their arguments are the same.  No segment contains
+hipMemsetAsync (all clears are fill kernels).  Graphs are captured through
+torch.cuda.CUDAGraph, which drives hipStreamBeginCapture.
 """
 import torch
 
@@ -20,6 +21,7 @@ class StepRunner:
         self.use_graph = use_graph
         self.stream = stream or torch.cuda.Stream(device=arena.device)
         self.graphs = None
+        self.split = False      # three graphs per step (pre / pair / post) while kernel events are on
         self._ev = None
         self._ev_i = 0
 
@@ -42,12 +44,17 @@ class StepRunner:
         self.ar.gather(self.stream)
         self.ar.nvt_step(self.stream)
 
+    def _whole(self):
+        self._pre()
+        self._pair()
+        self._post()
+
     def _capture(self):
         # buffers and kernel choice are settled before capture; capture launches nothing
         self.ar.prepare_pair()
         self.stream.synchronize()
         self.graphs = []
-        for seg in (self._pre, self._pair, self._post):
+        for seg in ((self._pre, self._pair, self._post) if self.split else (self._whole,)):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=self.stream):
                 seg()
@@ -76,9 +83,12 @@ class StepRunner:
             self._eager_step()
             return
         with torch.cuda.stream(self.stream):
-            self.graphs[0].replay()
-            self._timed(self.graphs[1].replay)
-            self.graphs[2].replay()
+            if self.split:
+                self.graphs[0].replay()
+                self._timed(self.graphs[1].replay)
+                self.graphs[2].replay()
+            else:
+                self.graphs[0].replay()
         self.ar.steps += 1
 
     def run(self, n):
@@ -97,9 +107,21 @@ class StepRunner:
 
     # timing -----------------------------------------------------------------
     def enable_kernel_events(self, n):
+        """Record events around the pair kernel of the next n steps (the step is recaptured as three
+        graphs while they are on)."""
+        if self.use_graph and not self.split:
+            self.split = True
+            self._capture()
         self._ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
         self._ev_i = 0
         return self._ev
+
+    def disable_kernel_events(self):
+        """Back to one graph per step."""
+        self._ev = None
+        if self.use_graph and self.split:
+            self.split = False
+            self._capture()
 
     def kernel_event_ms(self):
         """Average duration of the timed segment over the recorded steps (synchronises)."""

@@ -54,12 +54,19 @@ class EnergyNet:
         self.en_max = float(en_max)
         self.device = device
         self.packed = self._pack()
-        self.packed_half = self._pack_half()
         if precision == "auto":
-            precision = "f16x3" if self.packed_half is not None else "f32"
-        if precision == "f16x3" and self.packed_half is None:
-            raise ValueError("the fp16 split path needs a width whose 16-padding is a multiple of 32")
+            # the fp16 split needs bounded hidden activations (tanh); ReLU networks run in FP32
+            precision = "f16x3" if act == "tanh" and self._lowp_size("f16x3") > 0 else "f32"
+        if precision not in _lib.NET_PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_lib.NET_PRECISIONS)} or 'auto'")
+        if precision in ("bf16", "bf16x3") and act != "tanh":
+            # bf16 rounding moves pre-activations across zero, where ReLU's derivative jumps: dE/dG
+            # errors of 15-30 % (bf16) and 4-6 % (bf16x3) measured on 40->64->64 and 150-wide networks
+            raise ValueError("the bf16 forms are supported for tanh networks only; use f16x3 or f32 for ReLU")
         self.precision = precision
+        self.packed_half = self._pack_lowp(precision) if precision != "f32" else None
+        if precision != "f32" and self.packed_half is None:
+            raise ValueError(f"the {precision} path needs a width whose 16-padding is a multiple of 32")
         self.w_dev = torch.from_numpy(self.packed).to(device) if device is not None else None
         self.wh_dev = (torch.from_numpy(self.packed_half.view(np.int16)).to(device)
                        if device is not None and self.packed_half is not None else None)
@@ -106,19 +113,23 @@ class EnergyNet:
                                           out.ctypes.data_as(_lib.c_f32p)), "nepx_net_pack")
         return out
 
-    def _pack_half(self):
-        n = _lib.lib.nepx_net_packed_halves(self.d_in, self.width, self.n_hidden)
+    def _lowp_size(self, precision):
+        return _lib.lib.nepx_net_packed_lowp(_lib.NET_PRECISIONS[precision], self.d_in, self.width, self.n_hidden)
+
+    def _pack_lowp(self, precision):
+        """16-bit MFMA operands (fp16 or bf16, hi or hi/lo) in the kernels' operand order."""
+        n = self._lowp_size(precision)
         if n <= 0:
             return None
-        out = np.zeros(n, dtype=np.float16)
+        out = np.zeros(n, dtype=np.uint16)
         ws = [np.ascontiguousarray(W, dtype=np.float32) for W, _ in self.layers]
         WP = (_lib.c_f32p * len(ws))(*[w.ctypes.data_as(_lib.c_f32p) for w in ws])
-        _lib.check(_lib.lib.nepx_net_pack_half(self.d_in, self.width, self.n_hidden, WP,
-                                               out.ctypes.data_as(C.c_void_p)), "nepx_net_pack_half")
+        _lib.check(_lib.lib.nepx_net_pack_lowp(_lib.NET_PRECISIONS[precision], self.d_in, self.width, self.n_hidden,
+                                               WP, out.ctypes.data_as(C.c_void_p)), "nepx_net_pack_lowp")
         return out
 
     def with_precision(self, precision):
-        """A copy of this network run at another MFMA precision ('f32' or 'f16x3')."""
+        """A copy of this network run at another MFMA precision ('f32', 'f16x3', 'bf16', 'bf16x3')."""
         return EnergyNet(self.layers, self.act, self.en_min, self.en_max, self.device, precision)
 
     def cstruct(self):
@@ -128,7 +139,7 @@ class EnergyNet:
         n.en_min, n.en_max = self.en_min, self.en_max
         n.w_packed = self.w_dev.data_ptr()
         n.w_half = self.wh_dev.data_ptr() if self.wh_dev is not None else None
-        n.precision = _lib.NET_F16X3 if self.precision == "f16x3" else _lib.NET_F32
+        n.precision = _lib.NET_PRECISIONS[self.precision]
         return n
 
     def layers_f64(self):

@@ -18,7 +18,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "anisotropic-ml_amd")]
 
-PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_FP32_MFMA_TFLOPS = 157.3      # MI355X FP32 (vector and matrix)
+NET_DESC = {"f32": "MFMA f32",
+            "f16x3": "fp16 hi/lo split MFMA (3 products per FP32 product, FP32 accumulate)",
+            "bf16x3": "bf16 hi/lo split MFMA (3 products per FP32 product, FP32 accumulate)",
+            "bf16": "bf16 MFMA (one product, FP32 accumulate)"}
 PEAK_FP16_MFMA_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
@@ -58,47 +62,90 @@ def pair_flops(n_sites, hyp, D, widths):
     return 2 * n_sites * 6 * (nang + 1) * nlm + ffnn_flops_per_pair(D, widths)
 
 
-def cpu_baseline(spec, shape_name, phi, hyp, net, target_s=15.0):
-    """Reference algorithm (10-row FD stencil, FP64 numpy, one thread) on a bounded pair sample."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _cpu_workers():
+    """Worker processes for the CPU baseline: the cores this process may run on, at most 16 (the host
+    share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
+    """The reference algorithm (KD-tree list every 100 steps, 10-row FD stencil, NEP descriptor, FFNN,
+    clip, scatter, NVT; FP64 numpy) on the host cores, pair chunks spread over worker processes with
+    one BLAS thread each.  Systems of <= 4096 bodies run whole steps of the workload itself; larger
+    ones time a sample of the workload's own pairs plus its list build and integrator."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
     import numpy as np
     from oracle import nep_oracle as O, reference_step as R
     sh = O.shape_table()[shape_name]
     layers = net.layers_f64()
-    # sample: pairs of the workload itself when small, else of a 600-body box at its packing
-    # fraction (same per-pair work)
-    from nepx.configs import make_system
-    small = spec if spec.n <= 4096 else make_system(shape_name, 600, phi, seed=123)
-    pairs = O.brute_pairs(small.x, small.L, sh["cutoff"])
-    from threadpoolctl import threadpool_limits
-    rng = np.random.default_rng(0)
-    done = 0
-    chunk = 1000
-    order = rng.permutation(len(pairs))
-    with threadpool_limits(limits=1):          # the BLAS calls (network layers) on one thread too
+    workers = _cpu_workers()
+    info = {"cores": workers, "os_cpu_count": os.cpu_count(), "cpu_model": _cpu_model(),
+            "blas_threads_per_worker": 1, "kind": "port", "unit": "particle-steps/s"}
+    state = dict(x=spec.x.copy(), v=spec.v.copy(), q=spec.q.copy(), p=spec.p.copy(), m=spec.mass.copy())
+    thermo = O.thermo_update(dict(xi=-0.0317711, eta=0.108609, xi_r=0.00814227, eta_r=-0.014238),
+                             0.5 * float(np.sum(spec.mass[:, None] * spec.v ** 2)),
+                             float(np.sum(O.rot_ke(spec.q, spec.p, spec.moi))), spec.ndof, spec.rndof, spec.dt,
+                             spec.tau, spec.kT)
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as pool:
+        list(pool.map(R.fd_chunk, [(spec.x[:2], spec.q[:2], np.zeros((0, 2), np.int64), sh, spec.L, hyp, layers,
+                                    "tanh", 0.0, 1.0, 1e-5, 1e-5)] * workers))     # start the workers
+        if spec.n <= 4096:
+            run = lambda st, th, n: R.reference_run(st, th, sh, spec.L, hyp, layers, spec.dt, spec.kT, spec.tau,
+                                                    spec.moi, spec.ndof, spec.rndof, n, pool=pool,
+                                                    chunk=max(64, 5000 // workers))
+            t0 = time.perf_counter()
+            state, thermo, _ = run(state, thermo, 1)                     # one step sizes the sample
+            steps = int(min(1000, max(2, target_s / (time.perf_counter() - t0))))
+            t0 = time.perf_counter()
+            state, thermo, tm = run(state, thermo, steps)
+            wall = time.perf_counter() - t0
+            builds = (steps + R.LIST_EVERY - 1) // R.LIST_EVERY
+            # one list build per 100 steps, as in the reference, whatever the sample length
+            per_step = (tm["force_s"] + tm["integrate_s"]) / steps + tm["list_s"] / builds / R.LIST_EVERY
+            return {**info, "value": spec.n / per_step,
+                    "sample": f"{steps} steps of the workload itself through the reference algorithm (KD-tree "
+                              f"list within cutoff + 2 every 100 steps, {tm['list_s'] / builds * 1e3:.1f} ms per "
+                              f"build; COM filter; 10-row FD stencil, NEP descriptor, FFNN, clip, scatter on "
+                              f"{workers} worker processes; FP64 NVT) in {wall:.1f} s"}
         t0 = time.perf_counter()
-        while time.perf_counter() - t0 < target_s and done < len(pairs):
-            pc = pairs[order[done:done + chunk]]
-            clouds, t = O.pair_clouds(small.x, small.q, pc, sh["sites"], small.L)
-            O.fd_pair_forces(clouds, len(sh["sites"]), t, sh["nep_cutoff"], hyp, layers, "tanh", 0.0, 1.0,
-                             1e-5, 1e-5)
-            done += len(pc)
+        nlist = R.kdtree_list(spec.x, spec.L, sh["cutoff"] + R.LIST_SKIN)
+        t_list = time.perf_counter() - t0
+        pairs = R.com_filter(spec.x, spec.L, nlist, sh["cutoff"])
+        sample = pairs[np.random.default_rng(0).permutation(len(pairs))]
+        chunk = 500
+        done = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < target_s and done < len(sample):
+            batch = sample[done:done + chunk * workers]
+            R.reference_forces(spec.x, spec.q, sh, spec.L, hyp, layers, pairs=batch, pool=pool, chunk=chunk)
+            done += len(batch)
         t_pairs = time.perf_counter() - t0
-        # integrator per particle (vectorised FP64 numpy)
-        th = dict(xi=0.0, eta=0.0, xi_r=0.0, eta_r=0.0, theta=1.0, e_r=1.0)
-        st = dict(x=spec.x, v=spec.v, q=spec.q, p=spec.p, m=spec.mass)
-        t1 = time.perf_counter()
-        O.nvt_step(st, np.zeros_like(spec.x), np.zeros_like(spec.x), spec.dt, spec.kT, spec.tau, spec.L,
-                   spec.moi, th)
-        per_particle = (time.perf_counter() - t1) / spec.n
-    per_pair = t_pairs / done
-    # pairs per particle of the full workload (half list) at this density
-    p_per_particle = len(pairs) / small.n
-    step_time = spec.n * (p_per_particle * per_pair + per_particle)
-    return {"value": spec.n / step_time, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{done} pairs of a {small.n}-body {shape_name} box at the same density through the reference "
-                      f"algorithm (10-row forward-FD stencil, NEP descriptor, FFNN, clip; FP64 numpy, 1 thread) "
-                      f"in {t_pairs:.1f} s, scaled by pairs per particle ({p_per_particle:.1f}); FP64 NVT "
-                      f"update timed on the full state"}
+    F0 = np.zeros_like(spec.x)
+    t1 = time.perf_counter()
+    O.nvt_step(state, F0, F0, spec.dt, spec.kT, spec.tau, spec.L, spec.moi, thermo)
+    t_int = time.perf_counter() - t1
+    step_time = len(pairs) * t_pairs / done + t_int + t_list / R.LIST_EVERY
+    return {**info, "value": spec.n / step_time,
+            "sample": f"{done} of the workload's {len(pairs)} pairs through the reference algorithm (10-row "
+                      f"FD stencil, NEP descriptor, FFNN, clip, scatter; FP64 numpy on {workers} worker "
+                      f"processes) in {t_pairs:.1f} s, scaled to all pairs; KD-tree list of the whole system "
+                      f"({t_list:.2f} s per build, one per 100 steps) and the FP64 NVT update timed on the full "
+                      f"state"}
 
 
 def serve_bench(a):
@@ -145,8 +192,7 @@ def serve_bench(a):
                                "(D=60) tanh",
                    "window_s": a.serve_seconds, "capacity_per_class": a.serve_capacity, "chunk": a.serve_chunk,
                    "jobs_per_class": {c: sum(j.cls == c for j in jobs) for c in classes}},
-        "dtype": f"f32 pair math ({'fp16 hi/lo split MFMA' if nets[40].precision == 'f16x3' else 'MFMA f32'}), "
-                 "f64 integrator",
+        "dtype": f"f32 pair math ({NET_DESC[nets[40].precision]}), f64 integrator",
         "data": "synthetic",
     }
     if a.serve_plain_window:
@@ -197,7 +243,7 @@ def ffnn_bench(a):
         "metric": "energy-network rows/s (forward + backward to the descriptor, 32k-tetrahedron step)",
         "value": P / (ms * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (fp16 hi/lo split MFMA, FP32 accumulate)" if half else "f32 (MFMA f32)",
+        "dtype": f"f32 ({NET_DESC[net.precision]})",
         "data": "synthetic (uniform [0, 4) descriptors, random-init FFNN)",
         "config": {"workload": f"{P} rows, FFNN {D}->{widths[0]}->{widths[1]}->1 tanh, nepx_ffnn_fwd_bwd"},
         "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP32_MFMA_TFLOPS,
@@ -260,9 +306,90 @@ def replicas_bench(a):
         "config": {"workload": "256 cube8 systems, N log-uniform [64, 4096], phi U[0.2, 0.5], kT U[0.5, 2], "
                                "rc 2.5, hypers (11, 6, 4), FFNN 40->64->64->1 tanh, hipGraph steps",
                    "particles": n_tot, "systems": len(specs)},
-        "dtype": f"f32 pair math ({'fp16 hi/lo split MFMA' if net.precision == 'f16x3' else 'MFMA f32'}), "
-                 "f64 integrator",
+        "dtype": f"f32 pair math ({NET_DESC[net.precision]}), f64 integrator",
         "data": "synthetic", "tuning": tuning,
+    }
+
+
+def long_run_bench(a):
+    """A long NVT run of the headline system (32,768 tetrahedra, phi 0.45, kT 1, dt 0.001; --long-steps
+    steps, default 10,000) with production steps, checked every 100 steps: exits non-zero at the first
+    list overflow or non-finite thermostat / energy record.  Reports the translational and rotational
+    temperatures 2 K_t / N_dof and 2 K_r / RN_dof (means and spread over the second half, against kT)
+    and the Nose-Hoover extended energy K_t + K_r + U + N_dof kT (tau^2 xi^2/2 + eta)
+    + RN_dof kT (tau^2 xi_r^2/2 + eta_r) along the run."""
+    import numpy as np
+    import torch
+    import nepx
+    from nepx import _lib
+    from nepx.configs import WORKLOADS, make_system
+    from nepx.engine import StepRunner
+    w = WORKLOADS["tet32k"]
+    shape = nepx.get_shape(w["shape"])
+    hyp = shape.hypers
+    spec = make_system(w["shape"], w["n"], w["phi"], kT=w["kT"], dt=w["dt"], seed=2024)
+    dev = torch.device("cuda:0")
+    net = nepx.EnergyNet.random(40, [64, 64], seed=7, device=dev, precision=a.precision)
+    ar = nepx.Arena([spec], shape, hyp, net, device=dev)
+    runner = StepRunner(ar, use_graph=True)
+    runner.init()
+    chunk, n_chunks = 100, max(1, a.long_steps // 100)
+    rows = []
+    t0 = time.perf_counter()
+    for c in range(n_chunks):
+        runner.run(chunk)
+        runner.stream.synchronize()
+        f = ar.flags()
+        th = ar.thermo()[0]
+        rec = {k: float(th[k]) for k in ("xi", "eta", "xi_r", "eta_r", "Kt", "Kr", "U", "steps")}
+        if f[_lib.FLAG_OVERFLOW]:
+            print(json.dumps({"error": f"neighbour list overflow by step {(c + 1) * chunk}", "last": rec}))
+            sys.exit(3)
+        if not all(np.isfinite(v) for v in rec.values()):
+            print(json.dumps({"error": f"non-finite state by step {(c + 1) * chunk}", "last": rec}))
+            sys.exit(4)
+        rows.append(rec)
+        if (c + 1) % 10 == 0:
+            print(f"[long] step {(c + 1) * chunk}: T_t {2 * rec['Kt'] / spec.ndof:.4f} "
+                  f"T_r {2 * rec['Kr'] / spec.rndof:.4f} U {rec['U']:.1f}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    x = ar.system_arrays(0)
+    if not all(np.all(np.isfinite(x[k])) for k in ("x", "v", "q", "p")):
+        print(json.dumps({"error": "non-finite particle state at the end"}))
+        sys.exit(4)
+    kT, tau = spec.kT, spec.tau
+    Tt = np.array([2 * r["Kt"] / spec.ndof for r in rows])
+    Tr = np.array([2 * r["Kr"] / spec.rndof for r in rows])
+    H = np.array([r["Kt"] + r["Kr"] + r["U"] + spec.ndof * kT * (0.5 * tau * tau * r["xi"] ** 2 + r["eta"])
+                  + spec.rndof * kT * (0.5 * tau * tau * r["xi_r"] ** 2 + r["eta_r"]) for r in rows])
+    half = len(rows) // 2
+    steps = np.array([r["steps"] for r in rows])
+    slope = float(np.polyfit(steps, H, 1)[0]) if len(rows) > 1 else 0.0
+    K0 = rows[0]["Kt"] + rows[0]["Kr"]
+    return {
+        "metric": "long NVT run: temperatures and extended-energy drift (32k tetrahedra)",
+        "value": float(Tt[half:].mean() / kT), "unit": "<T_t>/kT over the second half",
+        "n_gpus": 1, "steps": int(steps[-1]), "higher_is_better": None,
+        "T_trans": {"mean_2nd_half": float(Tt[half:].mean()), "std_2nd_half": float(Tt[half:].std()),
+                    "first": float(Tt[0]), "last": float(Tt[-1])},
+        "T_rot": {"mean_2nd_half": float(Tr[half:].mean()), "std_2nd_half": float(Tr[half:].std()),
+                  "first": float(Tr[0]), "last": float(Tr[-1])},
+        "kT": kT,
+        "extended_energy": {"first": float(H[0]), "last": float(H[-1]), "min": float(H.min()),
+                            "max": float(H.max()), "slope_per_1000_steps": 1000.0 * slope,
+                            "drift_over_kinetic": float((H[-1] - H[0]) / K0),
+                            "note": "K at the half-step velocities of the record, U of the step's pairs; pair "
+                                    "energies do not vanish at the COM cutoff, so pairs crossing it change U "
+                                    "discontinuously"},
+        "samples": len(rows), "sample_every": chunk, "wall_s": wall,
+        "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc={shape.cutoff} hypers={hyp} D=40 "
+                               f"FFNN 40->64->64->1 tanh ({net.precision}), NVT kT={kT} dt={w['dt']} tau={tau}",
+                   "particles": spec.n, "list_builds": int(ar.flags()[_lib.FLAG_BUILDS])},
+        "series": {"step": steps[:: max(1, len(rows) // 20)].tolist(),
+                   "T_trans": Tt[:: max(1, len(rows) // 20)].tolist(),
+                   "T_rot": Tr[:: max(1, len(rows) // 20)].tolist(),
+                   "H": H[:: max(1, len(rows) // 20)].tolist()},
+        "data": "synthetic",
     }
 
 
@@ -275,9 +402,10 @@ def main():
     ap.add_argument("--autotune", type=int, default=1, choices=[0, 1, 2],
                     help="pick skin and launch shapes by timing candidates before the timed region "
                          "(1: use the cache when it has this workload, 2: always re-measure, 0: defaults)")
-    ap.add_argument("--precision", default="auto", choices=["auto", "f32", "f16x3"],
-                    help="network MFMA operands: FP32, or fp16 hi/lo split (3 products, FP32 accumulate)")
+    ap.add_argument("--precision", default="auto", choices=["auto"] + sorted(NET_DESC),
+                    help="network MFMA operands: " + "; ".join(f"{k}: {v}" for k, v in NET_DESC.items()))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--long-steps", type=int, default=10000, help="nvt_long workload: steps of the run")
     ap.add_argument("--graph", type=int, default=1, help="capture the step in a hipGraph (1) or launch eagerly (0)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="after the timed region, copy MIB MiB device-to-device once (a kernel of known traffic "
@@ -306,6 +434,9 @@ def main():
         return
     if a.workload == "replicas256":
         print(json.dumps(replicas_bench(a)))
+        return
+    if a.workload == "nvt_long":
+        print(json.dumps(long_run_bench(a)))
         return
 
     import numpy as np
@@ -337,24 +468,53 @@ def main():
     torch.cuda.synchronize()
     ar.check()
 
-    # timed region: K steps, events around every fused pair kernel on the step stream
+    # timed region: K production steps (one captured graph each), an event between steps for the
+    # per-step latency percentiles
     stream = runner.stream
-    fused_ev = runner.enable_kernel_events(a.steps)
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
+    from nepx import _lib
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
+    builds0 = int(ar.flags()[_lib.FLAG_BUILDS])
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
-    t_start.record(stream)
-    runner.run(a.steps)
-    t_end.record(stream)
+    ev[0].record(stream)
+    for k in range(a.steps):
+        runner.step()
+        ev[k + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     flags = ar.check()
-    ms = t_start.elapsed_time(t_end)
-    ms_step = ms / a.steps
-    fused_ms = runner.kernel_event_ms()
+    step_ms = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.steps)])
+    ms = ev[0].elapsed_time(ev[-1])
+    ms_window = ms / a.steps
+    builds_window = int(flags[_lib.FLAG_BUILDS]) - builds0
     P = int(flags[3])
     th = ar.thermo()[0]
+
+    # the pair kernel alone: the same steps with events around it (three graphs per step)
+    runner.enable_kernel_events(a.steps)
+    runner.run(a.steps)
+    fused_ms = runner.kernel_event_ms()
+    runner.disable_kernel_events()
+
+    # list rebuilds: cost of a rebuild over the no-op test every step runs, and how often the skin
+    # runs out (untimed probe of production steps); a window without a rebuild is charged its
+    # expected share, one with rebuilds its own
+    def timed_build(force, n=3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(n):
+            ar.build_list(force, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+    rebuild_ms = timed_build(True) - timed_build(False)
+    probe = max(200, 4 * a.steps)
+    b0 = int(ar.flags()[_lib.FLAG_BUILDS])
+    runner.run(probe)
+    torch.cuda.synchronize()
+    rate = (int(ar.check()[_lib.FLAG_BUILDS]) - b0) / probe
+    ms_step = ms_window + (rate * a.steps - builds_window) * rebuild_ms / a.steps
 
     import ctypes
     lanes = int(nepx._lib.lib.nepx_pair_fused_supported(ctypes.byref(ar.model), ctypes.byref(ar.cnet)))
@@ -365,10 +525,9 @@ def main():
                if lanes == 4 else "FP32 VALU (descriptor, adjoint) + MFMA (network); the bound is FP32 compute")
     flops = pair_flops(shape.n_sites, hyp, D, widths) * P
     achieved = flops / (fused_ms * 1e-3) / 1e12
-    half = net.precision == "f16x3"
     peak = PEAK_FP32_MFMA_TFLOPS
-    mfma_desc = "fp16 hi/lo split MFMA (3 products per FP32 product, FP32 accumulate)" if half else "MFMA f32"
-    prof = profiled(a.workload)
+    mfma_desc = NET_DESC[net.precision]
+    prof = profiled(a.workload) if net.precision == "f16x3" else profiled(None)
     res = {
         "metric": ("particle-steps/s at 32k tetrahedra" if a.workload == "tet32k"
                    else f"particle-steps/s ({a.workload})"),
@@ -381,13 +540,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "ms_per_step_window": ms_window,
+        "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
+                            "max": float(step_ms.max())},
+        "list_rebuild": {"ms": rebuild_ms, "per_step": rate, "in_window": builds_window,
+                         "note": "ms_per_step and value charge the window rebuild_ms x per_step x steps in place "
+                                 "of the rebuilds that fell inside it"},
         "dtype": f"f32 pair math ({mfma_desc}), f64 integrator",
         "data": "synthetic (seeded lattice + random orientations, random-init FFNN)",
         "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc={shape.cutoff} hypers={hyp} D={D} "
-                               f"FFNN {D}->{widths[0]}->{widths[1]}->1 tanh, NVT kT={w['kT']} dt={w['dt']}",
+                               f"FFNN {D}->{widths[0]}->{widths[1]}->1 tanh ({net.precision}), NVT kT={w['kT']} "
+                               f"dt={w['dt']}",
                    "particles": spec.n, "pairs_per_step": P, "hipgraph": bool(a.graph),
                    "list_builds": int(flags[4]), "wall_s": wall},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": prof["traffic"],
                      "kernel": kernel, "kernel_ms": fused_ms, "compute": compute,
                      "flops_per_launch": flops, "flops_per_pair": pair_flops(shape.n_sites, hyp, D, widths),
@@ -407,7 +573,7 @@ def main():
         torch.cuda.synchronize()
         res["copy_calib_bytes"] = {"read": 4 * n, "write": 4 * n}
     if not a.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(spec, w["shape"], w["phi"], hyp, net)
+        res["cpu_baseline"] = cpu_baseline(spec, w["shape"], hyp, net)
     print(json.dumps(res))
 
 

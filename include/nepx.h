@@ -47,12 +47,15 @@ typedef struct nepx_net {
     float en_min, en_max;            /* E = en_min + (en_max-en_min) * NN(G)    */
     const float* w_packed;           /* nepx_net_pack output (FP32 operands, biases, output layer) */
     const void* w_half;              /* nepx_net_pack_half output, or NULL        */
-    int32_t precision;               /* NEPX_NET_F32 or NEPX_NET_F16X3           */
+    int32_t precision;               /* NEPX_NET_F32, _F16X3, _BF16 or _BF16X3   */
     int32_t pad;
 } nepx_net;
 
 #define NEPX_NET_F32 0               /* v_mfma_f32_16x16x4_f32                    */
 #define NEPX_NET_F16X3 1             /* hi/lo fp16 split, 3 x v_mfma_f32_16x16x32_f16, FP32 accumulate */
+#define NEPX_NET_BF16 2              /* bf16 operands, 1 x v_mfma_f32_16x16x32_bf16, FP32 accumulate */
+#define NEPX_NET_BF16X3 3            /* hi/lo bf16 split, 3 x v_mfma_f32_16x16x32_bf16, FP32 accumulate */
+/* The bf16 forms take tanh networks only (NEPX_EUNSUP for ReLU). */
 
 /* Per-system parameters (device array of n_sys records, 16 doubles + 16 ints). */
 typedef struct nepx_sysparam {
@@ -155,6 +158,12 @@ int nepx_net_pack(int32_t d_in, int32_t width, int32_t n_hidden, const float* co
 /* fp16 hi/lo operands for NEPX_NET_F16X3 (width a multiple of 32); count in halves. */
 int64_t nepx_net_packed_halves(int32_t d_in, int32_t width, int32_t n_hidden);
 int nepx_net_pack_half(int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
+                       uint16_t* host_out);
+/* 16-bit operands for any precision other than NEPX_NET_F32 (fp16 or bf16, hi only or hi/lo;
+ * bf16 rounded to nearest even), in the same operand order; count in 16-bit elements.  Passed as
+ * nepx_net.w_half.  Same replacement as nepx_net_pack (ForceEvaluator.py:55-61). */
+int64_t nepx_net_packed_lowp(int32_t precision, int32_t d_in, int32_t width, int32_t n_hidden);
+int nepx_net_pack_lowp(int32_t precision, int32_t d_in, int32_t width, int32_t n_hidden, const float* const* ws,
                        uint16_t* host_out);
 
 /* ---- descriptor, energy network, adjoint ------------------------------- */

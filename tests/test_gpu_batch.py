@@ -38,3 +38,40 @@ def test_batched_equals_alone(gpu):
             assert np.array_equal(a[key], b[key]), (k, key, float(np.abs(a[key] - b[key]).max()))
         ta, tb = alone.thermo()[0], batched.thermo()[k]
         assert ta["xi"] == tb["xi"] and ta["Kt"] == tb["Kt"] and ta["U"] == tb["U"]
+
+
+def test_cube_replica_arena_equals_alone(gpu):
+    """The batched-replica workload: 256 cube systems, N log-uniform in [64, 4096], packing fraction
+    uniform in [0.2, 0.5], kT uniform in [0.5, 2.0] (seeded), stepped together with captured graphs;
+    the smallest, the largest, the densest and two seeded others equal their runs alone bitwise."""
+    import nepx
+    from nepx.configs import replica_specs
+    from nepx.engine import StepRunner
+    specs = replica_specs(256, seed=2024)
+    n = np.array([len(s.x) for s in specs])
+    dens = np.array([len(s.x) / s.L ** 3 for s in specs])
+    assert n.min() < 100 and n.max() > 3000 and n.sum() > 150_000
+    pick = sorted({int(n.argmin()), int(n.argmax()), int(dens.argmax())} |
+                  set(np.random.default_rng(7).choice(256, 2, replace=False).tolist()))
+    net = nepx.EnergyNet.random(40, [64, 64], seed=13, device=gpu)
+    shape = nepx.get_shape("cube8")
+    steps = 20
+    batched = nepx.Arena(specs, shape, HYP, net, device=gpu)
+    rb = StepRunner(batched, use_graph=True)
+    rb.init()
+    rb.run(steps)
+    torch.cuda.synchronize()
+    batched.check()
+    assert batched.device_steps() == steps
+    print(f"cube replicas: {len(specs)} systems, {n.sum()} particles; checking systems {pick}")
+    for k in pick:
+        alone = nepx.Arena([specs[k]], shape, HYP, net, device=gpu)
+        ra = StepRunner(alone, use_graph=True)
+        ra.init()
+        ra.run(steps)
+        torch.cuda.synchronize()
+        a, b = alone.system_arrays(0), batched.system_arrays(k)
+        for key in ("x", "v", "q", "p", "F", "T"):
+            assert np.array_equal(a[key], b[key]), (k, key, float(np.abs(a[key] - b[key]).max()))
+        ta, tb = alone.thermo()[0], batched.thermo()[k]
+        assert ta["xi"] == tb["xi"] and ta["Kt"] == tb["Kt"] and ta["U"] == tb["U"]

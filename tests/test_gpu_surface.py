@@ -185,3 +185,33 @@ def test_sim_gsd_dump_and_exact_resume(gpu, tmp_path):
     c.run(10)
     err = float(np.abs(c.central_pos.cpu().numpy() - a.central_pos.cpu().numpy()).max())
     assert err < 1e-9, err
+
+
+def test_sim_dump_labels_follow_device_steps_after_overflow(gpu, tmp_path):
+    """A run that starts with an overflowed list (its steps are skipped until the host grows the
+    list): every frame is labelled with the device's step count, labels are strictly increasing, and
+    each frame holds the state an unconstrained run has at that step."""
+    from nepx import gsd
+    net = _net(gpu)
+    ref = _nvt(gpu, net)
+    ref.setGsdDump(5, str(tmp_path / "ref.gsd"))
+    ref.run(30)
+    a = _nvt(gpu, net)
+    a.setGsdDump(5, str(tmp_path / "ovf.gsd"))
+    a.move_data_to_device()
+    a.arena._alloc_list(64)                # far below the ~570 pairs in range: the forced build overflows
+    a.arena.build_list(True)
+    a.runner._capture()
+    assert a.arena.flags()[1] == 1
+    a.run(30)
+    n = gsd.n_frames(str(tmp_path / "ovf.gsd"))
+    assert n == gsd.n_frames(str(tmp_path / "ref.gsd")) == 6
+    steps = []
+    for k in range(n):
+        fa = gsd.read_frame(str(tmp_path / "ovf.gsd"), k)
+        fr = gsd.read_frame(str(tmp_path / "ref.gsd"), k)
+        steps.append(int(fa["configuration/step"][0]))
+        assert steps[-1] == int(fr["configuration/step"][0])
+        assert np.array_equal(fa["log/nepx/position_f64"], fr["log/nepx/position_f64"]), steps[-1]
+    assert steps == [0, 5, 10, 15, 20, 25]
+    assert a.arena.device_steps() == 30

@@ -93,38 +93,61 @@ def test_net_packing_reproduces_dense_network(d_in, width, n_hidden, act):
     assert np.allclose(dG, g64, rtol=1e-4, atol=1e-5)
 
 
-def _split16(x):
-    hi = x.astype(np.float16)
-    return hi.astype(np.float64), (x - hi.astype(np.float32)).astype(np.float16).astype(np.float64)
+def _bf16_round(x):
+    """float32 -> bf16 value (as float64), round to nearest even, by integer bit arithmetic."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-def _emulate_f16x3_net(half, packed, d_in, width, n_hidden, G, act):
-    """Run the fp16 hi/lo operands through the 16x16x32 f16 MFMA lane layout on the CPU.
+def _split16(x, precision="f16x3"):
+    """hi and lo parts of float32 x as the device forms them (lo = 0 for plain bf16)."""
+    if precision == "f16x3":
+        hi = x.astype(np.float16)
+        return hi.astype(np.float64), (x - hi.astype(np.float32)).astype(np.float16).astype(np.float64)
+    hi = _bf16_round(x)
+    if precision == "bf16":
+        return hi, np.zeros_like(hi)
+    return hi, _bf16_round((x - hi.astype(np.float32)).astype(np.float32))
+
+
+def _operand_values(buf, precision):
+    if precision == "f16x3":
+        return buf.view(np.float16).astype(np.float64)
+    return (buf.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _emulate_lowp_net(buf, packed, d_in, width, n_hidden, G, act, precision="f16x3"):
+    """Run the 16-bit operands through the 16x16x32 MFMA lane layout on the CPU.
 
     A operand lane l, element j: A[l%16][8*(l/16) + j]; B operand likewise with
     the column l%16.  K-step s of a layer maps (lane group g, j) to unit
-    16*(2s + j/4) + 4g + j%4.  The activations are split into fp16 hi/lo parts
-    exactly as on the device (deltas after scaling by 2^10) and the products
-    hi*hi + hi*lo + lo*hi are summed, so both the operand layout and the
-    precision of the split are checked.
+    16*(2s + j/4) + 4g + j%4.  The activations are split into hi/lo parts
+    exactly as on the device (fp16: deltas after scaling by 2^10; bf16: no
+    scaling, and no lo part for plain bf16) and the products hi*hi + hi*lo +
+    lo*hi are summed, so both the operand layout and the precision of the split
+    are checked.
     """
+    parts = 1 if precision == "bf16" else 2
+    up = 1024.0 if precision == "f16x3" else 1.0
     DP, HP, NL = (d_in + 15) // 16 * 16, (width + 15) // 16 * 16, n_hidden + 1
     DPH = (DP + 31) // 32 * 32
-    fsz = [2 * (HP // 16) * ((DPH if k == 0 else HP) // 32) * 512 for k in range(NL)]
-    bsz = [2 * ((DP if k == 0 else HP) // 16) * (HP // 32) * 512 for k in range(NL)]
+    fsz = [parts * (HP // 16) * ((DPH if k == 0 else HP) // 32) * 512 for k in range(NL)]
+    bsz = [parts * ((DP if k == 0 else HP) // 16) * (HP // 32) * 512 for k in range(NL)]
     foff = [sum(fsz[:k]) for k in range(NL)]
     boff = [sum(fsz) + sum(bsz[:k]) for k in range(NL)]
+    assert len(buf) == sum(fsz) + sum(bsz)
     # fp32 layout: biases and the output layer
     f32 = [(HP // 16) * ((DP if k == 0 else HP) // 4) * 64 for k in range(NL)]
     b32 = [((DP if k == 0 else HP) // 16) * (HP // 4) * 64 for k in range(NL)]
     bias_off = [sum(f32) + sum(b32) + k * HP for k in range(NL)]
     wout = bias_off[-1] + HP
-    H = half.astype(np.float64)
+    H = _operand_values(buf, precision)
 
     def matmul(off, M, K, X, sc):
         ks = K // 32
         lo = (M // 16) * ks * 512
-        xh, xl = _split16((X * sc).astype(np.float32))
+        xh, xl = _split16((X * sc).astype(np.float32), precision)
         out = np.zeros((M, X.shape[1]))
         for mt in range(M // 16):
             for s in range(ks):
@@ -133,7 +156,9 @@ def _emulate_f16x3_net(half, packed, d_in, width, n_hidden, G, act):
                         e = off + ((mt * ks + s) * 64 + lane) * 8 + j
                         row = 16 * mt + lane % 16
                         u = 16 * (2 * s + j // 4) + 4 * (lane // 16) + j % 4
-                        out[row] += H[e] * xh[u] + H[e] * xl[u] + H[e + lo] * xh[u]
+                        out[row] += H[e] * xh[u]
+                        if parts == 2:
+                            out[row] += H[e] * xl[u] + H[e + lo] * xh[u]
         return out / sc
 
     Gp = np.zeros((G.shape[0], DPH))
@@ -148,22 +173,51 @@ def _emulate_f16x3_net(half, packed, d_in, width, n_hidden, G, act):
     y = packed[wout:wout + HP] @ h + packed[wout + HP]
     delta = packed[wout:wout + HP][:, None] * ders[-1]
     for k in range(NL - 1, 0, -1):
-        delta = matmul(boff[k], HP, HP, delta, 1024.0) * ders[k - 1]
-    dG = matmul(boff[0], DP, HP, delta, 1024.0)
+        delta = matmul(boff[k], HP, HP, delta, up) * ders[k - 1]
+    dG = matmul(boff[0], DP, HP, delta, up)
     return y, dG[:d_in].T
 
 
+# error budgets against FP64 of the emulated 16-bit forms (E, dE/dG); plain bf16 rounds every
+# operand to 8 mantissa bits
+LOWP_TOL = {"f16x3": (1e-5, 1e-4), "bf16x3": (1e-5, 1e-4), "bf16": (2e-2, 5e-2)}
+
+
+@pytest.mark.parametrize("precision", ["f16x3", "bf16x3", "bf16"])
 @pytest.mark.parametrize("d_in,width,n_hidden,act", [(40, 64, 1, "tanh"), (60, 128, 1, "tanh")])
-def test_half_net_packing_and_precision(d_in, width, n_hidden, act):
+def test_lowp_net_packing_and_precision(d_in, width, n_hidden, act, precision):
     import nepx
-    net = nepx.EnergyNet.random(d_in, [width] * (n_hidden + 1), seed=5, act=act, device=None)
-    assert net.precision == "f16x3"
+    net = nepx.EnergyNet.random(d_in, [width] * (n_hidden + 1), seed=5, act=act, device=None, precision=precision)
+    if precision == "f16x3":
+        assert nepx.EnergyNet.random(d_in, [width] * (n_hidden + 1), seed=5, act=act, device=None).precision == "f16x3"
     rng = np.random.default_rng(1)
     G = rng.uniform(0, 2, (4, d_in)) * rng.choice([1.0, 1e-3], size=d_in)   # include small descriptor entries
-    y, dG = _emulate_f16x3_net(net.packed_half, net.packed.astype(np.float64), d_in, width, n_hidden, G, act)
+    y, dG = _emulate_lowp_net(net.packed_half, net.packed.astype(np.float64), d_in, width, n_hidden, G, act,
+                              precision)
     y64, g64 = O.ffnn_forward_backward(G, net.layers_f64(), act)
-    assert np.abs(y - y64).max() < 1e-5 * np.abs(y64).max()
-    assert np.abs(dG - g64).max() < 1e-4 * np.abs(g64).max()
+    tE, tG = LOWP_TOL[precision]
+    eE = np.abs(y - y64).max() / np.abs(y64).max()
+    eG = np.abs(dG - g64).max() / np.abs(g64).max()
+    print(f"{precision} {d_in}->{width}: E {eE:.2e} dG {eG:.2e}")
+    assert eE < tE and eG < tG
+
+
+def test_bf16_operands_round_to_nearest_even():
+    """Host packing rounds each weight to the nearest bf16 (ties to even); the lo part of the split
+    form holds the rounded remainder."""
+    import nepx
+    W = np.zeros((32, 32), dtype=np.float32)
+    W[0, :4] = [1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, 1.0 + 2.0 ** -9, -3.14159265]
+    layers = [(W, np.zeros(32)), (np.eye(32, dtype=np.float32), np.zeros(32)), (np.ones((1, 32)), np.zeros(1))]
+    a = nepx.EnergyNet(layers, device=None, precision="bf16").packed_half
+    b = nepx.EnergyNet(layers, device=None, precision="bf16x3").packed_half
+    va, vb = _operand_values(a, "bf16"), _operand_values(b, "bf16x3")
+    # forward operand of layer 0, M tile 0, k-step 0: lane 0 holds row 0, columns j = 0..7
+    assert va[0] == 1.0 and va[1] == 1.0 + 4 * 2.0 ** -8 and va[2] == 1.0          # ties to even
+    assert va[3] == -3.140625
+    lo_off = 2 * 1 * 512                    # the lo part follows (HP/16) * (DPH/32) * 512 hi elements
+    assert vb[lo_off + 0] == 2.0 ** -8 and vb[lo_off + 1] == -(2.0 ** -8) and vb[lo_off + 2] == 2.0 ** -9
+    assert np.array_equal(_bf16_round(W[0, :4]), va[:4])
 
 
 def _quad_layout(nang, lmax, n_sites, f, Z, B):
@@ -470,3 +524,52 @@ def test_label_host_helpers_match_oracle():
     ref = np.concatenate([qx[:, 3:], qx[:, :3]], axis=1)
     assert np.abs(np.abs(np.sum(q * ref, axis=1)) - 1.0).max() < 1e-12
     assert np.abs(label.cube_structure_beads() - O.cube_structure_beads()).max() == 0.0
+
+
+REF_MODELS = "/root/reference/nep_simulator/models"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_MODELS), reason="reference checkpoints are only in the build container")
+@pytest.mark.parametrize("name,hyp", [("model_th_C104", (10, 8, 4)), ("model_th_C100", (10, 10, 10)),
+                                      ("model_nep_C104", (10, 8, 4)), ("model_nep_C100", (10, 10, 10))])
+def test_reference_checkpoint_packing(name, hyp):
+    """ForceEvaluator.read_model on the reference's own TheEnergyNet(150, 3, D) checkpoints
+    (ForceEvaluator.py:55-61, loaded with the weights-only loader): the packed MFMA operands, run
+    through the documented lane layout, reproduce the checkpoint's module in FP64 (energy and
+    dE/dG) on descriptor rows of the reference's initial configuration (th_init_N200.gsd, th_v1)."""
+    import torch
+    import nepx
+    from nepx.ForceEvaluator import ForceEvaluator
+    from nepx.GsdHandler import GsdHandler
+    D = nepx.n_descriptor(*hyp)
+    fe = ForceEvaluator()
+    fe.torch_device = None                      # pack on the host only
+    fe.setNN_dimensions(150, 3)
+    fe.setNd(D)
+    fe.read_model(os.path.join(REF_MODELS, name + ".pth"))
+    assert fe.net.precision == "f32" and fe.net.act == "relu" and fe.net.n_hidden == 3
+    h = GsdHandler(os.path.join(os.path.dirname(__file__), "golden", "th_init_N200.gsd"))
+    x, q, L = h.get_central_pos().astype(np.float64), h.getOrientations().astype(np.float64), h.getLx()
+    sh = O.shape_table()["th_v1"]
+    pairs = O.brute_pairs(x, L, sh["cutoff"])[:12]
+    clouds, _ = O.pair_clouds(x, q, pairs, sh["sites"], L)
+    G = O.descriptor(clouds, sh["nep_cutoff"], *hyp)
+    y, dG = _emulate_mfma_net(fe.net.packed.astype(np.float64), D, 150, 3, G, "relu")
+    mod = fe.energy_net.double()
+    Gt = torch.from_numpy(G).requires_grad_(True)
+    yt = mod(Gt)[:, 0]
+    (gt,) = torch.autograd.grad(yt.sum(), Gt)
+    y64, g64 = yt.detach().numpy(), gt.numpy()
+    assert np.abs(y - y64).max() < 1e-5 * np.abs(y64).max()
+    assert np.abs(dG - g64).max() < 1e-4 * np.abs(g64).max()
+    # a transposed first layer or a dropped hidden layer would not reproduce the module
+    assert np.abs(y64).max() > 0 and np.abs(g64).max() > 0
+
+
+def test_bf16_forms_refused_for_relu_networks():
+    import nepx
+    with pytest.raises(ValueError):
+        nepx.EnergyNet.random(40, [64, 64], seed=1, act="relu", device=None, precision="bf16")
+    with pytest.raises(ValueError):
+        nepx.EnergyNet.random(40, [64, 64], seed=1, act="relu", device=None, precision="bf16x3")
+    assert nepx.EnergyNet.random(40, [64, 64], seed=1, act="tanh", device=None, precision="bf16").precision == "bf16"
