@@ -10,6 +10,8 @@
 // Pair predicate (all paths, bit-exact with the oracle): t = X_j - X_i per
 // component, one wrap (t > L/2 -> t - L, t < -L/2 -> t + L), then
 // d2 = (tx*tx + ty*ty) + tz*tz in FP64 with no fused multiply-add, d2 < r*r.
+#include <mutex>
+
 #include "nepx_common.h"
 
 namespace nepx {
@@ -553,16 +555,24 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
     }
 }
 
-static int check_state(const nepx_state* st, const nepx_nlist* nl) {
+// State and list buffers every entry point reads; blk_count (the compaction's
+// per-block partials) is checked by the entry points that take it from the list.
+static int check_state_core(const nepx_state* st, const nepx_nlist* nl) {
     NEPX_REQUIRE(st && nl, NEPX_EINVAL, "null state or list");
     NEPX_REQUIRE(st->n > 0 && st->n_sys > 0, NEPX_EINVAL, "empty state");
     NEPX_REQUIRE(st->x && st->y && st->z && st->sys && st->sp, NEPX_EINVAL, "null state buffer");
     NEPX_REQUIRE(nl->flags && nl->nbr && nl->slot_row && nl->rnbr && nl->row_start && nl->rrow_start &&
-                     nl->row_count && nl->rrow_count && nl->cidx && nl->pair_i && nl->pair_j && nl->pair_t &&
-                     nl->blk_count,
+                     nl->row_count && nl->rrow_count && nl->cidx && nl->pair_i && nl->pair_j && nl->pair_t,
                  NEPX_EINVAL, "null list buffer");
     NEPX_REQUIRE(nl->cap > 0 && nl->cap < (int64_t)1 << 31, NEPX_ECAP, "list capacity %lld out of range",
                  (long long)nl->cap);
+    return NEPX_OK;
+}
+
+static int check_state(const nepx_state* st, const nepx_nlist* nl) {
+    const int rc = check_state_core(st, nl);
+    if (rc) return rc;
+    NEPX_REQUIRE(nl->blk_count, NEPX_EINVAL, "null list buffer (blk_count)");
     return NEPX_OK;
 }
 
@@ -650,6 +660,59 @@ extern "C" int nepx_pairs_filter(nepx_state* st, nepx_nlist* nl, void* stream) {
     hipLaunchKernelGGL(k_filter_count, dim3(nblk), dim3(kNB), 0, s, *st, *nl);
     hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(kScanBlock), 0, s, *nl, nblk);
     hipLaunchKernelGGL(k_filter_write, dim3(nblk), dim3(kNB), 0, s, *st, *nl);
+    NEPX_LAUNCH_CHECK();
+    return NEPX_OK;
+}
+
+// ---- allocation-free filter --------------------------------------------------
+// Scratch of nepx_pairs_filter_noalloc: the compaction's per-block partial counts
+// (one int per kSlotsPerBlock list slots, then the total), in place of the
+// list's blk_count.  A file-scope device array: each GPU the library runs on
+// holds its own copy (256 KiB) until the process ends.  Host code never passes
+// the array's name to a kernel; it uses the address hipGetSymbolAddress returns
+// for the current device, looked up once per device on the first call there.
+constexpr int kNoallocMaxBlocks = 1 << 16;   // list capacity up to 64 Mi slots
+static __device__ __align__(256) int32_t scratch[kNoallocMaxBlocks + 1];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_for_current_device(int32_t** out) {
+    int dev = -1;
+    NEPX_HIP_CHECK(hipGetDevice(&dev));
+    NEPX_REQUIRE(dev >= 0 && dev < kMaxDevices, NEPX_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev] == nullptr) {
+        void* p = nullptr;
+        NEPX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NEPX_REQUIRE(p != nullptr, NEPX_EHIP, "no device address for the filter scratch");
+        g_scratch_dev[dev] = static_cast<int32_t*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return NEPX_OK;
+}
+}  // namespace
+
+extern "C" int nepx_pairs_filter_noalloc(nepx_state* st, nepx_nlist* nl, void* stream) {
+    int rc = check_state_core(st, nl);
+    if (rc) return rc;
+    const int nblk = blocks_for(nl->cap, kSlotsPerBlock);
+    NEPX_REQUIRE(nblk <= kNoallocMaxBlocks, NEPX_ECAP,
+                 "list capacity %lld exceeds the %lld slots of the allocation-free filter's scratch",
+                 (long long)nl->cap, (long long)kNoallocMaxBlocks * kSlotsPerBlock);
+    int32_t* scr = nullptr;
+    rc = scratch_for_current_device(&scr);
+    if (rc) return rc;
+    // k_filter_count writes every partial it uses (zero for empty blocks) and k_filter_scan
+    // the total, so what an earlier call left in the scratch is never read.
+    nepx_nlist l = *nl;
+    l.blk_count = scr;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_filter_count, dim3(nblk), dim3(kNB), 0, s, *st, l);
+    hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(kScanBlock), 0, s, l, nblk);
+    hipLaunchKernelGGL(k_filter_write, dim3(nblk), dim3(kNB), 0, s, *st, l);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }

@@ -112,6 +112,7 @@ def _load():
         "nepx_nlist_build": ([C.POINTER(State), C.POINTER(NList), C.c_int32, vp], C.c_int),
         "nepx_nlist_from_pairs": ([C.POINTER(State), C.POINTER(NList), vp, C.c_int64, vp], C.c_int),
         "nepx_pairs_filter": ([C.POINTER(State), C.POINTER(NList), vp], C.c_int),
+        "nepx_pairs_filter_noalloc": ([C.POINTER(State), C.POINTER(NList), vp], C.c_int),
         "nepx_forces_gather": ([C.POINTER(State), C.POINTER(NList), vp, vp], C.c_int),
         "nepx_nvt_step": ([C.POINTER(State), C.POINTER(NList), vp], C.c_int),
         "nepx_nvt_init": ([C.POINTER(State), C.POINTER(NList), C.c_int32, vp], C.c_int),

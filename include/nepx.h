@@ -244,6 +244,15 @@ int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64_t* pairs, 
 /* Per-step COM filter d2 < rc^2 (FP64) and compaction into pair_i/pair_j/pair_t;
  * count in flags[3].  Replaces DescriptorGenerator.py:168-185. */
 int nepx_pairs_filter(nepx_state* st, nepx_nlist* nl, void* stream);
+/* nepx_pairs_filter without a workspace from the caller: the compaction's per-block
+ * partial counts live in a static device array of the library (one per GPU, 256 KiB,
+ * held until the process ends) instead of nl->blk_count, which may be NULL; every
+ * other buffer is the caller's, as for nepx_pairs_filter, and the results are the
+ * same.  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); it calls no allocating routine.  Limit: nl->cap <= 64 Mi slots
+ * (NEPX_ECAP beyond).  Calls that overlap on one GPU share the array and race. */
+int nepx_pairs_filter_noalloc(nepx_state* st, nepx_nlist* nl, void* stream);
 /* Deterministic gather of per-pair outputs into F, T, upot (FP64).
  * Replaces the index_add_ scatter (ForceEvaluator.py:135-142). */
 int nepx_forces_gather(nepx_state* st, nepx_nlist* nl, const float* pair_out, void* stream);
