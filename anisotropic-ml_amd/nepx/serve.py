@@ -95,6 +95,33 @@ class SlotArena(Arena):
         if list_cap is None:
             list_cap = 200 * self._capacity   # half-list slots per particle at the densest supported packing
         super().__init__([], shape, hypers, net, device=device, clip=clip, list_cap=list_cap)
+        # device bytes per particle, per list slot and per cell of this arena's buffers
+        n, cap = self.n, self.cap
+        self.particle_bytes = (sum(v.numel() * v.element_size() for k, v in self.t.items()
+                                   if k not in ("sp", "th", "partials")) +
+                               sum(self.lt[k].numel() * self.lt[k].element_size()
+                                   for k in ("cell_list", "cell_of", "cell_pos", "row_count", "row_start",
+                                             "rrow_count", "rrow_start"))) / n
+        self.slot_bytes = (sum(self.lt[k].numel() * self.lt[k].element_size()
+                               for k in ("nbr", "slot_row", "rnbr", "cidx", "pair_i", "pair_j", "pair_t")) +
+                           self.pair_out.numel() * self.pair_out.element_size()) / cap
+        self.cell_bytes = sum(self.lt[k].numel() * self.lt[k].element_size()
+                              for k in ("cell_count", "cell_start", "cell_fill")) / (self.n_cells + 1)
+
+    def device_bytes(self):
+        """Bytes of device memory the arena holds (state, list, records, per-pair outputs)."""
+        return (sum(v.numel() * v.element_size() for v in self.t.values()) +
+                sum(v.numel() * v.element_size() for v in self.lt.values()) +
+                self.pair_out.numel() * self.pair_out.element_size())
+
+    def job_bytes(self, spec):
+        """Device bytes a job occupies while resident: its block-aligned particle range, its share of
+        list slots (estimated pairs within the list radius) and its cells."""
+        nblk, pairs = self.footprint(spec)
+        r_list = spec.rc + spec.skin
+        nc = int(math.floor(spec.L / r_list))
+        ncell = nc ** 3 if nc >= 3 else 0
+        return int(nblk * BLOCK * self.particle_bytes + pairs * self.slot_bytes + ncell * self.cell_bytes)
 
     def _layout(self):
         self.n = self._capacity
@@ -255,17 +282,32 @@ class _ClassRunner:
 
 
 class Server:
-    """Continuous batching: one SlotArena per pair-model class, classes on separate streams."""
+    """Continuous batching: one SlotArena per pair-model class, classes on separate streams.
 
-    def __init__(self, classes, capacity, max_systems, chunk=20, device="cuda"):
+    Admission is by memory: a job is admitted only while the device bytes of all resident jobs
+    (SlotArena.job_bytes: particle range, share of list slots, cells) plus its own stay within
+    `mem_budget`, and it finds a free particle range, list share and cell range in its class's
+    arena.  The arenas are allocated up front (arena_bytes, reported); their capacities are what keeps
+    the GPU busy, since more resident particles than that only lengthen every step."""
+
+    def __init__(self, classes, capacity, max_systems, chunk=20, device="cuda", mem_budget=200 * 2 ** 30):
         # classes: name -> (Shape, hypers, EnergyNet)
         self.device = torch.device(device)
         self.classes = {}
         for name, (shape, hyp, net) in classes.items():
             ar = SlotArena(shape, hyp, net, capacity, max_systems, device=self.device)
             self.classes[name] = _ClassRunner(ar)
+        self.mem_budget = int(mem_budget)
+        self.arena_bytes = sum(cr.arena.device_bytes() for cr in self.classes.values())
+        self.resident_bytes = 0
+        self.peak_resident_bytes = 0
+        self.job_bytes = {}
         self.chunk = chunk
         self.recoveries = 0
+
+    def _fits_budget(self, cr, job):
+        b = cr.arena.job_bytes(job.spec)
+        return self.resident_bytes + b <= self.mem_budget, b
 
     def run(self, jobs, time_limit=None, log=None):
         queue = collections.deque(sorted(jobs, key=lambda j: j.arrival))
@@ -286,15 +328,22 @@ class Server:
             # admission (FIFO per class; a job that does not fit blocks the ones behind it); all device
             # work of a class, copies included, stays on that class's stream
             for name, cr in self.classes.items():
-                while waiting[name] and not cr.arena.fits_ever(waiting[name][0]):
+                while waiting[name] and (not cr.arena.fits_ever(waiting[name][0]) or
+                                         cr.arena.job_bytes(waiting[name][0].spec) > self.mem_budget):
                     j = waiting[name].popleft()
-                    j.result = {"error": "job exceeds the arena capacity"}
+                    j.result = {"error": "job exceeds the arena capacity or the memory budget"}
                     rejected.append(j)
                 with torch.cuda.stream(cr.runner.stream):
                     admitted = False
-                    while waiting[name] and cr.arena.admit(waiting[name][0]):
+                    while waiting[name]:
+                        ok, b = self._fits_budget(cr, waiting[name][0])
+                        if not ok or not cr.arena.admit(waiting[name][0]):
+                            break
                         j = waiting[name].popleft()
                         j.admitted_at = now
+                        self.job_bytes[j.jid] = b
+                        self.resident_bytes += b
+                        self.peak_resident_bytes = max(self.peak_resident_bytes, self.resident_bytes)
                         admitted = True
                     if admitted:
                         cr.arena.commit_admissions(cr.runner.stream)
@@ -316,6 +365,7 @@ class Server:
                 for j in retired:
                     j.done_at = time.perf_counter() - t0
                     retired_ps += j.n * j.steps
+                    self.resident_bytes -= self.job_bytes.pop(j.jid)
                     finished.append(j)
             if log and now - last_log > 5.0:
                 last_log = now
@@ -338,6 +388,9 @@ class Server:
         st = _stats("served", finished, retired_ps + resident_ps, wall, lat)
         st["jobs_rejected"] = len(rejected)
         st["list_regrowths"] = self.recoveries
+        st["mem_budget_bytes"] = self.mem_budget
+        st["arena_bytes"] = self.arena_bytes
+        st["peak_resident_job_bytes"] = self.peak_resident_bytes
         return st
 
 

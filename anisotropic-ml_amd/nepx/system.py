@@ -153,7 +153,7 @@ class Arena:
         for sp in self.specs:
             rho = sp.n / sp.L ** 3
             tot += 0.5 * sp.n * min(sp.n - 1, rho * 4.0 / 3.0 * math.pi * (sp.rc + sp.skin) ** 3)
-        return int(1.3 * tot) + 4096
+        return int(2.0 * tot) + 4096    # structure can pack pairs well above the uniform-density count
 
     def _alloc_list(self, cap=None):
         dev, n = self.device, self.n

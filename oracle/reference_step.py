@@ -55,7 +55,8 @@ def reference_forces(X, Q, shape, L, hyp, layers, act="tanh", en_min=0.0, en_max
     pairs = O.brute_pairs(X, L, rc) if pairs is None else pairs
     N = len(X)
     jobs = [(X, Q, pc, shape, L, hyp, layers, act, en_min, en_max, dx, dteta) for pc in _chunks(pairs, chunk)]
-    parts = list(pool.map(fd_chunk, jobs)) if pool is not None else [fd_chunk(j) for j in jobs]
+    # a worker that stops answering fails the map instead of holding it up
+    parts = list(pool.map(fd_chunk, jobs, timeout=600)) if pool is not None else [fd_chunk(j) for j in jobs]
     if parts:
         Fi, Ti, Tj = (np.concatenate([p[k] for p in parts]) for k in range(3))
     else:
