@@ -390,15 +390,15 @@ __device__ __forceinline__ void desc_finish(const float (&A)[Cfg::NANG + 1][LR::
 // range LR, c_lm = sum_n B_nlm f_n(rho_a) and d_lm = sum_n B_nlm f_n'(rho_a).
 // Adds sum_lm d_lm Z_lm to dEdrho and returns
 //   (gx, gy, gz) = grad sum_lm c_lm Z_lm  (Z_lm extended to homogeneous polynomials of degree l)
-//   proj         = sum_lm l c_lm Z_lm(u_a) = u_a . (gx, gy, gz)  (Euler's theorem for the
-//                  homogeneous extension of degree l)
+//   proj         = sum_lm l c_lm Z_lm(u_a),
 // using  d/dx Q_l^m = x Q_{l-1}^{m+1},  d/dy Q_l^m = y Q_{l-1}^{m+1},  d/dz Q_l^m = (l+m) Q_{l-1}^m,
 //        d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}),
 // with the sums over l taken first for each m, so each (l, m) costs a few FMAs.
 template <class LR, class CD>
-__device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, float uz, CD&& cd,
+__device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, CD&& cd,
                                                   float& dEdrho, float& gx, float& gy, float& gz, float& proj) {
     float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f;
+    proj = 0.f;
 #pragma unroll
     for (int mm = 0; mm <= LR::L1; ++mm) {
         float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
@@ -407,6 +407,7 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
             const float ql = h.q(l, mm);
             const float qa = (mm + 1 <= l - 1) ? h.q(l - 1, mm + 1) : 0.f;
             const float qc = (mm <= l - 1) ? (float)(l + mm) * h.q(l - 1, mm) : 0.f;
+            float pl = 0.f;
 #pragma unroll
             for (int part = 0; part < (mm ? 2 : 1); ++part) {
                 const int o = lm_slot(l, mm, part) - LR::OFF;
@@ -414,10 +415,12 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
                 float c, d;
                 cd(o, c, d);
                 dEdrho += d * z;
+                pl += c * z;
                 if (mm + 1 <= l - 1) U[part] += qa * c;
                 if (mm <= l - 1) V[part] += qc * c;
                 if (mm >= 1) W[part] += ql * c;
             }
+            proj += (float)l * pl;
         }
         if (mm == 0) {
             E1 += U[0];
@@ -433,7 +436,6 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
     gx = ux * E1 + X1;
     gy = uy * E1 + Y1;
     gz = Gz;
-    proj = ux * gx + uy * gy + uz * gz;
 }
 
 // dE/dr_a = dE/drho_a u_a / r_nep + (grad - proj u_a) / |r_a|.
@@ -508,7 +510,7 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
             Harm<LR::L1> h;
             h.eval(s.ux, s.uy, s.uz);
             float gx, gy, gz, proj;
-            angular_site_grad<LR>(h, s.ux, s.uy, s.uz,
+            angular_site_grad<LR>(h, s.ux, s.uy,
                                   [&](int o, float& c, float& d) {
                                       c = 0.f;
                                       d = 0.f;

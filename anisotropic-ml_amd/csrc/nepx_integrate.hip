@@ -86,11 +86,17 @@ __device__ __forceinline__ double disp2(const nepx_state& st, int i, double L) {
 // 5 = kinetic energies + displacement for the thermostat tail.
 enum { M_STEP = 0, M_KE = 1, M_TWO = 2, M_TRANS = 3, M_ROT = 4, M_THERMO = 5 };
 
-// With `flags` set and the list overflowed, the step is skipped as a whole
-// (uniformly: every thread reads the same flag) so the state stays at the
-// last good step until the host has grown the list.
-__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, const int32_t* __restrict__ flags) {
-    if (flags && flags[1]) return;
+__device__ void reduce_system(nepx_state& st, int32_t* flags, int s, int update, int count_step);
+
+// With `flags` set and the list overflowed or the step frozen (flags[6]: the list
+// went stale inside a multi-step graph), the step is skipped as a whole
+// (uniformly: every thread reads the same flag) so the state stays at the last
+// good step until the list is grown or rebuilt.  In the whole-step mode the
+// last block of each system to write its partials (counted in the system's
+// record, sp.ipad[0]) also runs reduce_system for it, so the step needs no second
+// launch; the sums are taken in the same fixed order whichever block is last.
+__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, int32_t* __restrict__ flags) {
+    if (flags && (flags[1] || flags[6])) return;  // list overflow, or a frozen step (stale list)
     const int i = blockIdx.x * kIB + threadIdx.x;
     double kt = 0.0, kr = 0.0, u = 0.0, d2 = 0.0;
     if (i < st.n && st.sys[i] >= 0 && st.sp[st.sys[i]].active) {
@@ -179,16 +185,29 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, const int3
     }
     if (mode == M_STEP || mode == M_THERMO || mode == M_KE)  // uniform per launch
         block_reduce4(kt, kr, u, d2, st.partials + 4 * (size_t)blockIdx.x);
+    if (mode != M_STEP) return;
+    // the block's system: that of its first particle (systems start on block boundaries)
+    const int i0 = blockIdx.x * kIB;
+    const int s = i0 < st.n ? st.sys[i0] : -1;
+    if (s < 0 || !st.sp[s].active) return;  // uniform per block
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        __threadfence();  // this block's partials are visible before it is counted
+        nepx_sysparam& sp = st.sp[s];
+        const int nb = sp.blk_end - sp.blk_start;
+        last = atomicAdd(&sp.ipad[0], 1) == nb - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every other block's partials are visible to this block's loads
+    reduce_system(st, flags, s, 1, 1);
+    if (threadIdx.x == 0) st.sp[s].ipad[0] = 0;  // ready for the next step
 }
 
-// One block per system: fixed-order sums of the block partials, thermostat
-// update, list rebuild test.
-__global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* flags, int update, int count_step) {
-    const int s = blockIdx.x;
+// Fixed-order sums of system s's block partials, thermostat update, list rebuild
+// test and step count; run by one whole block.
+__device__ void reduce_system(nepx_state& st, int32_t* flags, int s, int update, int count_step) {
     nepx_sysparam& sp = st.sp[s];
-    if (!sp.active) return;  // uniform per block
-    if (flags && flags[1]) return;  // list overflow: the step was skipped
-    if (update == 2 && st.th[s].pending == 0.0) return;  // only newly admitted systems
     double a = 0, b = 0, c = 0, d = 0;
     for (int k = sp.blk_start + threadIdx.x; k < sp.blk_end; k += kIB) {
         const double* pp = st.partials + 4 * (size_t)k;
@@ -227,13 +246,25 @@ __global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* f
             th.theta = exp(-0.5 * th.xi * sp.dt);
             th.e_r = exp(-0.5 * sp.dt * th.xi_r);
         }
-        const double half = 0.5 * sp.skin;
+        // the list is stale once a particle has moved half the skin; with an early fraction f > 0
+        // (sp.pad[0]) a rebuild is requested at the next build point from f times that on
+        const double half = 0.5 * sp.skin, early = sp.pad[0] * half;
         if (flags && out[3] > half * half) flags[0] = 1;
+        else if (flags && early > 0.0 && out[3] > early * early) flags[7] = 1;
         if (count_step) {
             th.steps += 1.0;
             if (th.budget > 0.0 && th.steps >= th.budget) sp.active = 0;  // finished: frozen until harvested
         }
     }
+}
+
+// One block per system: reduce_system (the initialisation and part-by-part paths).
+__global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* flags, int update, int count_step) {
+    const int s = blockIdx.x;
+    if (!st.sp[s].active) return;  // uniform per block
+    if (flags && flags[1]) return;  // list overflow: the step was skipped
+    if (update == 2 && st.th[s].pending == 0.0) return;  // only newly admitted systems
+    reduce_system(st, flags, s, update, count_step);
 }
 
 }  // namespace nepx
@@ -253,8 +284,7 @@ extern "C" int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream) {
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0, nl ? (const int32_t*)nl->flags : nullptr);
-    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0, nl ? nl->flags : nullptr);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
@@ -264,7 +294,7 @@ extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_ther
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1, (const int32_t*)nullptr);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1, (int32_t*)nullptr);
     hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr,
                        update_thermostat == 2 ? 2 : (update_thermostat ? 1 : 0), 0);
     NEPX_LAUNCH_CHECK();
@@ -279,7 +309,7 @@ extern "C" int nepx_nvt_part(nepx_state* st, nepx_nlist* nl, int32_t part, void*
     NEPX_REQUIRE(st->dp, NEPX_EINVAL, "null dp buffer");
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part, nl ? (const int32_t*)nl->flags : nullptr);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part, nl ? nl->flags : nullptr);
     if (part == NEPX_NVT_THERMO)
         hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();

@@ -21,7 +21,7 @@ constexpr int kScanBlock = 1024;
 constexpr int kSortMax = 2048;  // longest row the in-LDS sort accepts
 constexpr int kSlotsPerBlock = 1024;
 
-enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5 };
+enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5, F_FROZEN = 6, F_SOON = 7 };
 
 __device__ __forceinline__ double wrap1(double t, double L) {
     t = t > 0.5 * L ? t - L : t;
@@ -48,7 +48,8 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nc) {
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
-__device__ __forceinline__ bool want_build(const int32_t* flags) { return flags[F_REBUILD] != 0; }
+// A build runs when the list is stale (F_REBUILD) or due soon (F_SOON, set while it is still valid).
+__device__ __forceinline__ bool want_build(const int32_t* flags) { return flags[F_REBUILD] != 0 || flags[F_SOON] != 0; }
 
 // A particle takes part in the list when it belongs to an active system.
 __device__ __forceinline__ bool live(const nepx_state& st, int i) {
@@ -332,8 +333,9 @@ __global__ __launch_bounds__(kNB) void k_rev_slots(nepx_nlist nl, int n) {
 }
 
 __global__ void k_build_finish(nepx_nlist nl) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && nl.flags[F_REBUILD]) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && want_build(nl.flags)) {
         nl.flags[F_REBUILD] = 0;
+        nl.flags[F_SOON] = 0;
         nl.flags[F_BUILDS] += 1;
     }
 }
@@ -419,10 +421,13 @@ __device__ __forceinline__ void eval_slots(const nepx_state& st, const nepx_nlis
     }
 }
 
+// A step whose list is stale (F_REBUILD still set: a step of a multi-step graph after the skin ran
+// out) is frozen: the filter selects no pairs and flags F_FROZEN, so the gather and the integrator
+// leave the state alone until the next build.
 __global__ __launch_bounds__(kNB) void k_filter_count(nepx_state st, nepx_nlist nl) {
     const int PL = nl.flags[F_PL];
     const int base = blockIdx.x * kSlotsPerBlock;
-    if (base >= PL || nl.flags[F_OVERFLOW]) {
+    if (base >= PL || nl.flags[F_OVERFLOW] || nl.flags[F_REBUILD]) {
         if (threadIdx.x == 0) nl.blk_count[blockIdx.x] = 0;
         return;
     }
@@ -441,21 +446,36 @@ __global__ __launch_bounds__(kNB) void k_filter_count(nepx_state st, nepx_nlist 
     if (threadIdx.x == 0) nl.blk_count[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_filter_scan(nepx_nlist nl, int nblk) {
-    // blk_count -> exclusive offsets in place via cidx-free scratch (blk_count[nblk] = total)
-    __shared__ int32_t tmp_total;
-    block_exclusive_scan(nl.blk_count, nl.blk_count, nblk);
+// Sum of cnt[0..b) by one block (integer, so the order does not matter).
+__device__ __forceinline__ int block_prefix(const int32_t* __restrict__ cnt, int b) {
+    __shared__ int wpart[kNB / 64];
+    int v = 0;
+    for (int k = threadIdx.x; k < b; k += kNB) v += cnt[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        tmp_total = nl.blk_count[nblk];
-        nl.flags[F_P] = nl.flags[F_OVERFLOW] ? 0 : tmp_total;
-    }
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < kNB / 64; ++w) t += wpart[w];
+    return t;
 }
 
+// Each block finds its output offset by summing the counts of the blocks before it (one count per
+// kSlotsPerBlock list slots, so few), and the block holding the last slot stores the pair count;
+// no separate scan launch.
 __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist nl) {
     const int PL = nl.flags[F_PL];
     const int base = blockIdx.x * kSlotsPerBlock;
-    if (base >= PL || nl.flags[F_OVERFLOW]) return;
+    const int frozen = nl.flags[F_REBUILD] != 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) nl.flags[F_FROZEN] = frozen;
+    if (nl.flags[F_OVERFLOW] || frozen || PL == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) nl.flags[F_P] = 0;
+        return;
+    }
+    if (base >= PL) return;
+    const int first = block_prefix(nl.blk_count, blockIdx.x);
+    if (base + kSlotsPerBlock >= PL && threadIdx.x == 0) nl.flags[F_P] = first + nl.blk_count[blockIdx.x];
     __shared__ int wsum[kSlotsPerThread][kNB / 64];
     SlotEval r;
     eval_slots(st, nl, base, PL, r);
@@ -468,7 +488,7 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
         if (lane == 0) wsum[k][wv] = __popcll(mask);
     }
     __syncthreads();
-    int out = nl.blk_count[blockIdx.x];
+    int out = first;
 #pragma unroll
     for (int k = 0; k < kSlotsPerThread; ++k) {
         const int s = base + k * kNB + threadIdx.x;
@@ -511,8 +531,9 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
     const int k = (blockIdx.x * kNB + threadIdx.x) / GL;
     const int sub = threadIdx.x % GL;
     if (k >= st.n) return;  // whole lane groups exit together
+    if (nl.flags[F_OVERFLOW] || nl.flags[F_FROZEN]) return;  // skipped step: forces stay as they are
     double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, u = 0;
-    if (live(st, k) && !nl.flags[F_OVERFLOW]) {
+    if (live(st, k)) {
         for (int s = nl.row_start[k] + sub; s < nl.row_start[k + 1]; s += GL) {
             const int c = nl.cidx[s];
             if (c < 0) continue;
@@ -658,7 +679,6 @@ extern "C" int nepx_pairs_filter(nepx_state* st, nepx_nlist* nl, void* stream) {
     hipStream_t s = as_stream(stream);
     const int nblk = blocks_for(nl->cap, kSlotsPerBlock);
     hipLaunchKernelGGL(k_filter_count, dim3(nblk), dim3(kNB), 0, s, *st, *nl);
-    hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(kScanBlock), 0, s, *nl, nblk);
     hipLaunchKernelGGL(k_filter_write, dim3(nblk), dim3(kNB), 0, s, *st, *nl);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
@@ -666,13 +686,13 @@ extern "C" int nepx_pairs_filter(nepx_state* st, nepx_nlist* nl, void* stream) {
 
 // ---- allocation-free filter --------------------------------------------------
 // Scratch of nepx_pairs_filter_noalloc: the compaction's per-block partial counts
-// (one int per kSlotsPerBlock list slots, then the total), in place of the
-// list's blk_count.  A file-scope device array: each GPU the library runs on
-// holds its own copy (256 KiB) until the process ends.  Host code never passes
-// the array's name to a kernel; it uses the address hipGetSymbolAddress returns
-// for the current device, looked up once per device on the first call there.
+// (one int per kSlotsPerBlock list slots), in place of the list's blk_count.
+// A file-scope device array: each GPU the library runs on holds its own copy
+// (256 KiB) until the process ends.  Host code never passes the array's name to
+// a kernel; it uses the address hipGetSymbolAddress returns for the current
+// device, looked up once per device on the first call there.
 constexpr int kNoallocMaxBlocks = 1 << 16;   // list capacity up to 64 Mi slots
-static __device__ __align__(256) int32_t scratch[kNoallocMaxBlocks + 1];
+static __device__ __align__(256) int32_t scratch[kNoallocMaxBlocks];
 
 namespace {
 constexpr int kMaxDevices = 64;
@@ -705,13 +725,12 @@ extern "C" int nepx_pairs_filter_noalloc(nepx_state* st, nepx_nlist* nl, void* s
     int32_t* scr = nullptr;
     rc = scratch_for_current_device(&scr);
     if (rc) return rc;
-    // k_filter_count writes every partial it uses (zero for empty blocks) and k_filter_scan
-    // the total, so what an earlier call left in the scratch is never read.
+    // k_filter_count writes every partial that k_filter_write reads (zero for empty blocks), so
+    // what an earlier call left in the scratch is never read.
     nepx_nlist l = *nl;
     l.blk_count = scr;
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_filter_count, dim3(nblk), dim3(kNB), 0, s, *st, l);
-    hipLaunchKernelGGL(k_filter_scan, dim3(1), dim3(kScanBlock), 0, s, l, nblk);
     hipLaunchKernelGGL(k_filter_write, dim3(nblk), dim3(kNB), 0, s, *st, l);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             h.eval(sg.ux, sg.uy, sg.uz);
             float gx, gy, gz, proj;
             angular_site_grad<LRange<1, LMAX>>(
-                h, sg.ux, sg.uy, sg.uz,
+                h, sg.ux, sg.uy,
                 [&](int o, float& c, float& dv) {
                     c = cc[o / 4][o % 4];
                     dv = dd[o / 4][o % 4];

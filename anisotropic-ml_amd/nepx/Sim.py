@@ -244,8 +244,9 @@ class Sim:
         last_dump = -1
         while self.timestep < Nsteps:
             if dumping and self.timestep % self.gsdDumpPeriod == 0 and self.timestep != last_dump:
-                # A list overflow freezes the state at its last good step while the host keeps
-                # counting; settle it before writing so every frame's label is the device's count.
+                # A list overflow (or a stale list inside a multi-step graph) freezes the state at its
+                # last good step while the host keeps counting; settle it before writing so every
+                # frame's label is the device's count.
                 torch.cuda.synchronize(self.device)
                 self._recover(base)
                 if self.timestep % self.gsdDumpPeriod == 0 and self.timestep != last_dump:
@@ -253,7 +254,11 @@ class Sim:
                     self.dumpConfig(self.timestep)
                     last_dump = self.timestep
                 continue
-            self.step()
+            # steps up to the next dump point, check point or the end, launched together
+            nxt = min(Nsteps, (self.timestep // check_every + 1) * check_every)
+            if dumping:
+                nxt = min(nxt, (self.timestep // self.gsdDumpPeriod + 1) * self.gsdDumpPeriod)
+            self._steps(nxt - self.timestep)
             if self.timestep % check_every == 0 or self.timestep == Nsteps:
                 self._recover(base)
         torch.cuda.synchronize(self.device)
@@ -270,11 +275,23 @@ class Sim:
         print("Done")
 
     def _recover(self, base):
-        """A list overflow makes the device skip steps; grow the list and resume from the device count."""
-        handled = self.runner.recover() if self.runner is not None else self.arena.recover_overflow()
-        if handled:
-            self.timestep = self.arena.device_steps() - base
+        """Synchronising: a list overflow makes the device skip steps (grow the list), a stale list
+        inside a multi-step graph freezes them; resume from the device's step count."""
+        if self.runner is not None:
+            self.runner.recover()
+        else:
+            self.arena.recover_overflow()
+        self.timestep = self.arena.device_steps() - base
         self.arena.check()
+
+    def _steps(self, m):
+        """Launch m steps (the host count advances by m; _recover reconciles it with the device)."""
+        if self.fd_compat or self.component_path:
+            for _ in range(m):
+                self.step()
+            return
+        self.runner.run(m)
+        self.timestep += m
 
     def step(self):
         self.descriptor_generator.fd_compat = self.fd_compat

@@ -70,7 +70,7 @@ THERMO_DTYPE = np.dtype([("xi", "f8"), ("eta", "f8"), ("xi_r", "f8"), ("eta_r", 
                          ("e_r", "f8"), ("Kt", "f8"), ("Kr", "f8"), ("U", "f8"), ("max_disp2", "f8"),
                          ("steps", "f8"), ("budget", "f8"), ("pending", "f8"), ("pad", "f8", 3)])
 
-FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW = range(6)
+FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW, FLAG_FROZEN, FLAG_SOON = range(8)
 NVT_STEP_TWO, NVT_TRANSLATE, NVT_ROTATE, NVT_THERMO = 2, 3, 4, 5
 N_FLAGS = 8
 

@@ -419,27 +419,29 @@ def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None)
             time.sleep(j.arrival - now)
         shape, hyp, net = classes[j.cls]
         ar = Arena([j.spec], shape, hyp, net, device=dev)
-        runner = StepRunner(ar, use_graph=True)
+        runner = StepRunner(ar, use_graph=True)      # multi-step graphs for small systems
         runner.init()
+        done_dev = 0
         left = j.steps
         while left > 0:
             if time_limit is not None and time.perf_counter() - t0 >= time_limit:
                 break
             k = min(chunk, left)
             st = runner.stream
-            evs = [torch.cuda.Event(enable_timing=True) for _ in range(k + 1)]
-            evs[0].record(st)
-            for i in range(k):
-                runner.step()
-                evs[i + 1].record(st)
+            ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ea.record(st)
+            runner.run(k)
+            eb.record(st)
             st.synchronize()
-            if runner.recover():          # list overflow: the device skipped steps; count what it did
+            if runner.recover():          # list overflow: the device skipped steps
                 regrowths += 1
-                k = ar.device_steps() - (j.steps - left)
-            else:
-                lat.extend(evs[i].elapsed_time(evs[i + 1]) for i in range(k))
-            left -= k
-            done_ps += j.n * k
+            now = ar.device_steps()       # frozen or skipped steps are not counted
+            kd = now - done_dev
+            done_dev = now
+            if kd > 0:
+                lat.extend([ea.elapsed_time(eb) / kd] * kd)
+            left -= kd
+            done_ps += j.n * kd
             if log and time.perf_counter() - t0 - last_log > 5.0:
                 last_log = time.perf_counter() - t0
                 log(last_log, {"job": j.jid, "n": j.n, "left": left}, len(finished))
@@ -470,19 +472,18 @@ def plain_step_costs(jobs, classes, device="cuda", steps=20, warmup=5, log=None)
         ar = Arena([copy.deepcopy(j.spec)], shape, hyp, net, device=dev)
         r = StepRunner(ar, use_graph=True)
         r.init()
-        r.run(warmup)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(r.stream)
-        r.run(steps)
-        b.record(r.stream)
-        r.stream.synchronize()
-        if r.recover():
+        r.run_exact(warmup)
+        for _ in range(3):
+            d0 = ar.device_steps()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(r.stream)
             r.run(steps)
             b.record(r.stream)
             r.stream.synchronize()
+            if not r.recover():
+                break
         ar.check()
-        costs[j.jid] = a.elapsed_time(b) / steps
+        costs[j.jid] = a.elapsed_time(b) / max(ar.device_steps() - d0, 1)
         del r, ar
         if log and (k + 1) % 16 == 0:
             log(time.perf_counter() - t0, {"plain_costs": k + 1}, len(jobs))

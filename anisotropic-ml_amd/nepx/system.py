@@ -52,6 +52,11 @@ class SystemSpec:
         return len(self.x)
 
 
+def _nullctx():
+    import contextlib
+    return contextlib.nullcontext()
+
+
 class Arena:
     """One or more systems sharing a pair model and energy network, resident on one GPU."""
 
@@ -360,6 +365,16 @@ class Arena:
     def tensor(self, keys, k=0):
         a, b = self.starts[k], self.starts[k] + self.specs[k].n
         return torch.stack([self.t[key][a:b] for key in keys], 1)
+
+    def set_early_rebuild(self, fraction, stream=None):
+        """Request list rebuilds at the next build point once a particle has moved `fraction` of half
+        the skin (0 turns this off; the list is rebuilt when it goes stale).  For multi-step graphs,
+        whose only build point is their first step: with an early request the list rarely goes stale
+        inside a graph, where the remaining steps would be frozen."""
+        rec = self.t["sp"].cpu().numpy().view(_lib.SYSPARAM_DTYPE).copy()
+        rec["pad"][:, 0] = float(fraction)
+        with torch.cuda.stream(stream) if stream is not None else _nullctx():
+            self.t["sp"].copy_(torch.from_numpy(rec.view(np.uint8).copy()))
 
     # --------------------------------------------------------------- access
     def thermo(self):

@@ -426,20 +426,9 @@ def long_run_bench(a):
 
 
 def run_recovering(runner, n):
-    """n steps that the device completes: a neighbour-list overflow makes the device skip steps; the
-    list is grown (runner.recover) and the missing steps are run.  Returns the overflows handled."""
-    ar = runner.ar
-    target = ar.device_steps() + n
-    handled = 0
-    left = n
-    while left > 0:
-        runner.run(left)
-        runner.stream.synchronize()
-        if runner.recover():
-            handled += 1
-        ar.check()
-        left = target - ar.device_steps()
-    return handled
+    """n steps that the device completes (StepRunner.run_exact): list overflows are grown and frozen
+    or skipped steps are run again.  Returns the overflows handled."""
+    return runner.run_exact(n)
 
 
 def main():
@@ -520,14 +509,18 @@ def main():
     # on the device) is discarded and run again after the list has grown
     stream = runner.stream
     from nepx import _lib
+    # (with multi-step graphs an event per graph: each of its steps is charged the graph's mean)
+    K = runner.K
+    launches = [K] * (a.steps // K) + [1] * (a.steps % K)
     for attempt in range(3):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(launches) + 1)]
         builds0 = int(ar.flags()[_lib.FLAG_BUILDS])
+        done0 = ar.device_steps()
         torch.cuda.synchronize()
         wall0 = time.perf_counter()
         ev[0].record(stream)
-        for k in range(a.steps):
-            runner.step()
+        for k, m in enumerate(launches):
+            runner.run_graph_k() if m > 1 else runner.step()
             ev[k + 1].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - wall0
@@ -535,9 +528,10 @@ def main():
             break
         overflows += 1
     flags = ar.check()
-    step_ms = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.steps)])
+    done = ar.device_steps() - done0          # frozen steps (stale list inside a graph) are not done
+    step_ms = np.concatenate([np.full(m, ev[k].elapsed_time(ev[k + 1]) / m) for k, m in enumerate(launches)])
     ms = ev[0].elapsed_time(ev[-1])
-    ms_window = ms / a.steps
+    ms_window = ms / max(done, 1)
     builds_window = int(flags[_lib.FLAG_BUILDS]) - builds0
     P = int(flags[3])
     th = ar.thermo()[0]
@@ -567,7 +561,7 @@ def main():
     b0 = int(ar.flags()[_lib.FLAG_BUILDS])
     overflows += run_recovering(runner, probe)
     rate = (int(ar.check()[_lib.FLAG_BUILDS]) - b0) / probe
-    ms_step = ms_window + (rate * a.steps - builds_window) * rebuild_ms / a.steps
+    ms_step = ms_window + (rate * done - builds_window) * rebuild_ms / max(done, 1)
 
     import ctypes
     lanes = int(nepx._lib.lib.nepx_pair_fused_supported(ctypes.byref(ar.model), ctypes.byref(ar.cnet)))
@@ -594,6 +588,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "ms_per_step_window": ms_window,
+        "steps_per_graph": K, "steps_done_in_window": done,
         "step_latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
                             "max": float(step_ms.max())},
         "list_rebuild": {"ms": rebuild_ms, "per_step": rate, "in_window": builds_window,

@@ -62,12 +62,14 @@ typedef struct nepx_sysparam {
     double L, kT, dt, tau, ndof, rndof;  /* box, temperature, step, NH tau, N_dof, RN_dof */
     double moi[3];                       /* diagonal inertia (Sim.py:43-44)      */
     double r_list, rc, skin;             /* list radius, COM cutoff, skin        */
-    double pad[4];
+    double pad[4];                       /* pad[0]: early-rebuild fraction f of the half skin (0 = off):
+                                            a rebuild is requested for the next build point once a
+                                            particle has moved f * skin / 2; others zero */
     int32_t p_start, p_end;              /* particle range (block aligned start) */
     int32_t blk_start, blk_end;          /* integrator block range               */
     int32_t nc, cell_off;                /* cells per side (0 = all pairs), offset */
     int32_t active;                      /* 0: the system is skipped by every kernel */
-    int32_t ipad[9];
+    int32_t ipad[9];                     /* zero; ipad[0] is the integrator's per-step block count */
 } nepx_sysparam;
 
 /* Thermostat record per system (Sim.py:99-111, IntegrateStepTwoKernels.py:74-130). */
@@ -117,8 +119,10 @@ typedef struct nepx_nlist {
     int32_t* slot_row;               /* cap: i of slot s                          */
     int32_t* rnbr;                   /* cap: reverse rows, then slot ids          */
     int32_t* cidx;                   /* cap: compacted index of slot or -1        */
-    int32_t* blk_count;              /* compaction partials (cap/1024+1)           */
-    int32_t* flags;                  /* [0] rebuild request, [1] overflow, [2] P_L, [3] P, [4] builds */
+    int32_t* blk_count;              /* compaction partials (cap/1024 rounded up)  */
+    int32_t* flags;                  /* [0] list stale (rebuild; steps wait), [1] overflow, [2] P_L,
+                                        [3] P, [4] builds, [5] longest row, [6] step frozen (set by the
+                                        filter while [0] is set), [7] rebuild at the next build point */
     int32_t* pair_i;                 /* cap: compacted pairs                       */
     int32_t* pair_j;
     float* pair_t;                   /* cap*4: minimum-image t = X_j - X_i (f32)  */
