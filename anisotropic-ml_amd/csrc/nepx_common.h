@@ -50,6 +50,19 @@ constexpr double cfact(int n) {
     return r;
 }
 
+// ---- XCD-contiguous block order ---------------------------------------------
+// The dispatcher deals a grid's blocks to the 8 XCDs in turn (block b to XCD b % 8),
+// and each XCD has its own L2.  Kernels whose neighbouring blocks share data take
+// their logical block from here: XCD x then works through one contiguous range of
+// logical blocks (the first nb % 8 XCDs take one block more), so the data that
+// neighbouring blocks share stays in one L2.  A bijection of [0, nb) for any nb.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_contiguous_block(int b, int nb) {
+    const int per = nb / kXcds, extra = nb % kXcds;
+    const int x = b % kXcds, r = b / kXcds;
+    return x * per + (x < extra ? x : extra) + r;
+}
+
 // ---- per-component force / torque clip -------------------------------------
 // v > c -> c, v < -c -> -c, anything else (including NaN) unchanged: the
 // masked assignments of ForceEvaluator.py:125-131.  c <= 0 disables the clip.

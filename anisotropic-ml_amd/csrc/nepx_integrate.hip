@@ -33,11 +33,26 @@ __device__ __forceinline__ Q4 perm(int k, const Q4& v) {
     return Q4{-v.z, v.y, -v.x, v.w};
 }
 
+// sin and cos of a rotor angle.  The angles are dt times a rotation rate, so small: below
+// 0.125 the Taylor series to x^13 (sin) and x^14 (cos) leaves a truncation error under 1e-25
+// relative, far below FP64 rounding; larger angles take the library routine.
+__device__ __forceinline__ void sincos_rotor(double x, double* s, double* c) {
+    if (fabs(x) < 0.125) {
+        const double x2 = x * x;
+        *s = x * (1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0 +
+                 x2 * (-1.0 / 39916800.0 + x2 * (1.0 / 6227020800.0)))))));
+        *c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24.0 + x2 * (-1.0 / 720.0 + x2 * (1.0 / 40320.0 +
+                 x2 * (-1.0 / 3628800.0 + x2 * (1.0 / 479001600.0 + x2 * (-1.0 / 87178291200.0)))))));
+    } else {
+        sincos(x, s, c);
+    }
+}
+
 __device__ __forceinline__ void rotor(int k, double h, double Ik, Q4& p, Q4& q) {
     const Q4 pp = perm(k, p), qq = perm(k, q);
     const double phi = (0.25 / Ik) * (p.w * qq.w + p.x * qq.x + p.y * qq.y + p.z * qq.z);
     double s, c;
-    sincos(h * phi, &s, &c);
+    sincos_rotor(h * phi, &s, &c);
     p = Q4{c * p.w + s * pp.w, c * p.x + s * pp.x, c * p.y + s * pp.y, c * p.z + s * pp.z};
     q = Q4{c * q.w + s * qq.w, c * q.x + s * qq.x, c * q.y + s * qq.y, c * q.z + s * qq.z};
 }
@@ -48,27 +63,36 @@ __device__ __forceinline__ double rot_ke(const Q4& q, const Q4& p, const double*
     return 0.5 * (sx * sx / moi[0] + sy * sy / moi[1] + sz * sz / moi[2]);
 }
 
+// Maximum that keeps a NaN (fmax would drop it): a non-finite displacement marks
+// the system as faulted in reduce_system.
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || a > b) ? a : b; }
+
+// Sums of a, b, c and the maximum of dmax (NaN kept) over the block, written to out[0..3] by
+// thread 0.  Butterfly sums within each 64-lane wave (every lane ends with the same,
+// order-fixed value: a + b == b + a exactly), then the four wave results in wave
+// order: one barrier, and the same bits on every run.
 __device__ void block_reduce4(double a, double b, double c, double dmax, double* out) {
-    __shared__ double red[4][kIB];
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
-    red[2][threadIdx.x] = c;
-    red[3][threadIdx.x] = dmax;
-    __syncthreads();
-    for (int off = kIB / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + off];
-            red[1][threadIdx.x] += red[1][threadIdx.x + off];
-            red[2][threadIdx.x] += red[2][threadIdx.x + off];
-            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + off]);
-        }
-        __syncthreads();
+    static_assert(kIB == 256, "four waves per block");
+    __shared__ double red[4][kIB / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+        c += __shfl_xor(c, o, 64);
+        dmax = nan_max(dmax, __shfl_xor(dmax, o, 64));
     }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = a;
+        red[1][w] = b;
+        red[2][w] = c;
+        red[3][w] = dmax;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        out[0] = red[0][0];
-        out[1] = red[1][0];
-        out[2] = red[2][0];
-        out[3] = red[3][0];
+        out[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        out[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        out[2] = (red[2][0] + red[2][1]) + (red[2][2] + red[2][3]);
+        out[3] = nan_max(nan_max(red[3][0], red[3][1]), nan_max(red[3][2], red[3][3]));
     }
 }
 
@@ -86,16 +110,14 @@ __device__ __forceinline__ double disp2(const nepx_state& st, int i, double L) {
 // 5 = kinetic energies + displacement for the thermostat tail.
 enum { M_STEP = 0, M_KE = 1, M_TWO = 2, M_TRANS = 3, M_ROT = 4, M_THERMO = 5 };
 
-__device__ void reduce_system(nepx_state& st, int32_t* flags, int s, int update, int count_step);
-
 // With `flags` set and the list overflowed or the step frozen (flags[6]: the list
 // went stale inside a multi-step graph), the step is skipped as a whole
 // (uniformly: every thread reads the same flag) so the state stays at the last
-// good step until the list is grown or rebuilt.  In the whole-step mode the
-// last block of each system to write its partials (counted in the system's
-// record, sp.ipad[0]) also runs reduce_system for it, so the step needs no second
-// launch; the sums are taken in the same fixed order whichever block is last.
-__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, int32_t* __restrict__ flags) {
+// good step until the list is grown or rebuilt.  Each block writes its partial
+// sums and k_reduce_thermo, a second launch, sums them per system (folding that
+// into the last block of each system needs a device-scope fence in every block,
+// which made this launch three times slower at 32,768 particles).
+__global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, const int32_t* __restrict__ flags) {
     if (flags && (flags[1] || flags[6])) return;  // list overflow, or a frozen step (stale list)
     const int i = blockIdx.x * kIB + threadIdx.x;
     double kt = 0.0, kr = 0.0, u = 0.0, d2 = 0.0;
@@ -185,23 +207,6 @@ __global__ __launch_bounds__(kIB) void k_nvt(nepx_state st, int mode, int32_t* _
     }
     if (mode == M_STEP || mode == M_THERMO || mode == M_KE)  // uniform per launch
         block_reduce4(kt, kr, u, d2, st.partials + 4 * (size_t)blockIdx.x);
-    if (mode != M_STEP) return;
-    // the block's system: that of its first particle (systems start on block boundaries)
-    const int i0 = blockIdx.x * kIB;
-    const int s = i0 < st.n ? st.sys[i0] : -1;
-    if (s < 0 || !st.sp[s].active) return;  // uniform per block
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        __threadfence();  // this block's partials are visible before it is counted
-        nepx_sysparam& sp = st.sp[s];
-        const int nb = sp.blk_end - sp.blk_start;
-        last = atomicAdd(&sp.ipad[0], 1) == nb - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // every other block's partials are visible to this block's loads
-    reduce_system(st, flags, s, 1, 1);
-    if (threadIdx.x == 0) st.sp[s].ipad[0] = 0;  // ready for the next step
 }
 
 // Fixed-order sums of system s's block partials, thermostat update, list rebuild
@@ -214,7 +219,7 @@ __device__ void reduce_system(nepx_state& st, int32_t* flags, int s, int update,
         a += pp[0];
         b += pp[1];
         c += pp[2];
-        d = fmax(d, pp[3]);
+        d = nan_max(d, pp[3]);
     }
     __shared__ double out[4];
     block_reduce4(a, b, c, d, out);
@@ -227,6 +232,13 @@ __device__ void reduce_system(nepx_state& st, int32_t* flags, int s, int update,
         th.U = out[2];
         th.max_disp2 = out[3];
         th.pending = 0.0;
+        if (!(isfinite(Kt) && isfinite(Kr) && isfinite(out[2]) && isfinite(out[3]))) {
+            // a non-finite state (a blown-up pair, an injected NaN): this system stops here and
+            // every kernel skips it from the next launch on; the other systems are untouched
+            th.fault = 1.0;
+            sp.active = 0;
+            return;
+        }
         if (update) {
             const double dt = sp.dt, tau = sp.tau, kT = sp.kT;
             const double cfac = 0.5 * ((dt / tau) / tau);
@@ -258,11 +270,11 @@ __device__ void reduce_system(nepx_state& st, int32_t* flags, int s, int update,
     }
 }
 
-// One block per system: reduce_system (the initialisation and part-by-part paths).
+// One block per system: reduce_system after every k_nvt that sums energies.
 __global__ __launch_bounds__(kIB) void k_reduce_thermo(nepx_state st, int32_t* flags, int update, int count_step) {
     const int s = blockIdx.x;
     if (!st.sp[s].active) return;  // uniform per block
-    if (flags && flags[1]) return;  // list overflow: the step was skipped
+    if (flags && (flags[1] || flags[6])) return;  // list overflow or frozen step: the step was skipped
     if (update == 2 && st.th[s].pending == 0.0) return;  // only newly admitted systems
     reduce_system(st, flags, s, update, count_step);
 }
@@ -284,7 +296,8 @@ extern "C" int nepx_nvt_step(nepx_state* st, nepx_nlist* nl, void* stream) {
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0, nl ? nl->flags : nullptr);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 0, nl ? (const int32_t*)nl->flags : nullptr);
+    hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
@@ -294,7 +307,7 @@ extern "C" int nepx_nvt_init(nepx_state* st, nepx_nlist* nl, int32_t update_ther
     if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1, (int32_t*)nullptr);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, 1, (const int32_t*)nullptr);
     hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, (int32_t*)nullptr,
                        update_thermostat == 2 ? 2 : (update_thermostat ? 1 : 0), 0);
     NEPX_LAUNCH_CHECK();
@@ -309,7 +322,7 @@ extern "C" int nepx_nvt_part(nepx_state* st, nepx_nlist* nl, int32_t part, void*
     NEPX_REQUIRE(st->dp, NEPX_EINVAL, "null dp buffer");
     hipStream_t s = as_stream(stream);
     const int nb = (st->n + kIB - 1) / kIB;
-    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part, nl ? nl->flags : nullptr);
+    hipLaunchKernelGGL(k_nvt, dim3(nb), dim3(kIB), 0, s, *st, (int)part, nl ? (const int32_t*)nl->flags : nullptr);
     if (part == NEPX_NVT_THERMO)
         hipLaunchKernelGGL(k_reduce_thermo, dim3(st->n_sys), dim3(kIB), 0, s, *st, nl ? nl->flags : nullptr, 1, 1);
     NEPX_LAUNCH_CHECK();

@@ -21,7 +21,8 @@ constexpr int kScanBlock = 1024;
 constexpr int kSortMax = 2048;  // longest row the in-LDS sort accepts
 constexpr int kSlotsPerBlock = 1024;
 
-enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5, F_FROZEN = 6, F_SOON = 7 };
+enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5, F_FROZEN = 6, F_SOON = 7,
+       F_LONGSYS = 8 };
 
 __device__ __forceinline__ double wrap1(double t, double L) {
     t = t > 0.5 * L ? t - L : t;
@@ -43,9 +44,13 @@ __device__ __forceinline__ double pair_d2(const nepx_state& st, double L, int i,
     return pos_d2(st.x[i], st.y[i], st.z[i], st.x[j], st.y[j], st.z[j], L, tx, ty, tz);
 }
 
+// Cell of coordinate x along one side, clamped to [0, nc): NaN and infinities (a faulted
+// system's particles, between the step that went non-finite and its removal) land in the
+// edge cells; the comparisons run in floating point so no out-of-range value is converted.
 __device__ __forceinline__ int cell_coord(double x, double L, int nc) {
-    int c = (int)floor((x + 0.5 * L) / L * (double)nc);
-    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+    const double f = floor((x + 0.5 * L) / L * (double)nc);
+    if (!(f >= 0.0)) return 0;
+    return f >= (double)nc ? nc - 1 : (int)f;
 }
 
 // A build runs when the list is stale (F_REBUILD) or due soon (F_SOON, set while it is still valid).
@@ -63,7 +68,10 @@ __global__ void k_set_flag(int32_t* flags, int idx, int v) {
 
 // Start of a build step: clear the longest-row flag and, when a rebuild is due, the cell counters.
 __global__ void k_build_begin(int32_t* flags, int32_t* cell_count, int32_t* cell_fill, int64_t nct) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) flags[F_MAXROW] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        flags[F_MAXROW] = 0;
+        flags[F_LONGSYS] = 0;
+    }
     if (!want_build(flags)) return;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nct; k += (int64_t)gridDim.x * blockDim.x) {
         cell_count[k] = 0;
@@ -224,7 +232,10 @@ __global__ __launch_bounds__(kNB) void k_nbr_count(nepx_state st, nepx_nlist nl)
     if (sub == 0) {
         nl.row_count[i] = nf;
         nl.rrow_count[i] = nr;
-        if (nf > kSortMax || nr > kSortMax) atomicMax(&nl.flags[F_MAXROW], nf > nr ? nf : nr);
+        if (nf > kSortMax || nr > kSortMax) {
+            atomicMax(&nl.flags[F_MAXROW], nf > nr ? nf : nr);
+            nl.flags[F_LONGSYS] = st.sys[i] + 1;  // one system holding such a row (any, if several)
+        }
     }
 }
 
@@ -528,7 +539,9 @@ __device__ __forceinline__ double red_lanes(double v) {
 
 template <int GL>
 __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, const float4* __restrict__ out) {
-    const int k = (blockIdx.x * kNB + threadIdx.x) / GL;
+    // particles of one XCD are contiguous, so the records that neighbouring rows share
+    // (forward rows of i, reverse rows of its neighbours) are fetched into one L2
+    const int k = (xcd_contiguous_block(blockIdx.x, gridDim.x) * kNB + threadIdx.x) / GL;
     const int sub = threadIdx.x % GL;
     if (k >= st.n) return;  // whole lane groups exit together
     if (nl.flags[F_OVERFLOW] || nl.flags[F_FROZEN]) return;  // skipped step: forces stay as they are
@@ -649,6 +662,7 @@ extern "C" int nepx_nlist_from_pairs(nepx_state* st, nepx_nlist* nl, const int64
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_REBUILD, 1);
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_OVERFLOW, 0);
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_MAXROW, 0);
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, nl->flags, F_LONGSYS, 0);
     hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(n + 1, kNB)), dim3(kNB), 0, s, nl->row_count,
                        (int64_t)(n + 1), nullptr);
     hipLaunchKernelGGL(k_zero_i32, dim3(blocks_for(n + 1, kNB)), dim3(kNB), 0, s, nl->rrow_count,

@@ -68,15 +68,26 @@ SYSPARAM_DTYPE = np.dtype([("L", "f8"), ("kT", "f8"), ("dt", "f8"), ("tau", "f8"
                            ("active", "i4"), ("ipad", "i4", 9)])
 THERMO_DTYPE = np.dtype([("xi", "f8"), ("eta", "f8"), ("xi_r", "f8"), ("eta_r", "f8"), ("theta", "f8"),
                          ("e_r", "f8"), ("Kt", "f8"), ("Kr", "f8"), ("U", "f8"), ("max_disp2", "f8"),
-                         ("steps", "f8"), ("budget", "f8"), ("pending", "f8"), ("pad", "f8", 3)])
+                         ("steps", "f8"), ("budget", "f8"), ("pending", "f8"), ("fault", "f8"),
+                         ("pad", "f8", 2)])
 
-FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW, FLAG_FROZEN, FLAG_SOON = range(8)
+FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW, FLAG_FROZEN, FLAG_SOON, FLAG_LONGSYS = range(9)
+MAX_ROW = 2048       # longest neighbour row the list build sorts
 NVT_STEP_TWO, NVT_TRANSLATE, NVT_ROTATE, NVT_THERMO = 2, 3, 4, 5
-N_FLAGS = 8
+N_FLAGS = 16
 
 
 class NepxError(RuntimeError):
     pass
+
+
+class RowLimitError(NepxError):
+    """A particle has more neighbours within the list radius than the build sorts (MAX_ROW):
+    the system has collapsed.  `system` is the index of one such system (-1: unknown)."""
+
+    def __init__(self, msg, system=-1):
+        super().__init__(msg)
+        self.system = system
 
 
 def _load():

@@ -47,10 +47,43 @@ class Job:
     done_at: float = -1.0
     steps_done: int = 0
     result: dict = field(default_factory=dict)
+    cancelled: bool = False
+    error: str = ""        # set when the server stops the job (e.g. its system collapsed)
 
     @property
     def n(self):
         return self.spec.n
+
+
+class LazySpec:
+    """A seeded system spec whose particle arrays are built on first use.  A long job trace
+    holds thousands of jobs, of which only those that reach the head of a queue need their
+    box and arrays; `release` drops the arrays again (they are rebuilt, identically, from the
+    seed if asked for later)."""
+
+    def __init__(self, shape_name, n, phi, **kw):
+        self._args = (shape_name, n, phi)
+        self._kw = kw
+        self._spec = None
+        self.n = n
+
+    def build(self):
+        from .configs import make_system
+        if self._spec is None:
+            self._spec = make_system(*self._args, **self._kw)
+        return self._spec
+
+    def release(self):
+        self._spec = None
+
+    def __getattr__(self, name):  # attributes other than n come from the built spec
+        if name.startswith("_"):
+            raise AttributeError(name)
+        return getattr(self.build(), name)
+
+
+def _spec_of(spec):
+    return spec.build() if isinstance(spec, LazySpec) else spec
 
 
 class _Intervals:
@@ -195,6 +228,8 @@ class SlotArena(Arena):
         self.blk_of = getattr(self, "blk_of", {})
         self.blk_of[slot] = (b0, nblk, c0, nc ** 3)
         job.slot = slot
+        if isinstance(spec, LazySpec):
+            spec.release()  # the state now lives on the device
         return True
 
     def _put_record(self, key, slot, rec):
@@ -224,12 +259,15 @@ class SlotArena(Arena):
             job.steps_done = int(th[slot]["steps"])
             if sp[slot]["active"] == 0:
                 a, b = int(sp[slot]["p_start"]), int(sp[slot]["p_end"])
-                if keep_state:
-                    g = lambda key: self.t[key][a:b].cpu().numpy()
-                    job.result = dict(x=np.stack([g("x"), g("y"), g("z")], 1),
-                                      v=np.stack([g("vx"), g("vy"), g("vz")], 1),
-                                      q=np.stack([g("q0"), g("q1"), g("q2"), g("q3")], 1),
-                                      p=np.stack([g("p0"), g("p1"), g("p2"), g("p3")], 1),
+                if th[slot]["fault"] != 0:
+                    job.result = {"error": f"non-finite state after {job.steps_done} steps; the job was "
+                                           "stopped at that step", "fault": True}
+                elif job.error:
+                    job.result = {"error": job.error}
+                elif job.cancelled:
+                    job.result = {"error": f"cancelled after {job.steps_done} steps"}
+                elif keep_state:
+                    job.result = dict(self.slot_state(sp[slot]),
                                       thermo={k: float(th[slot][k]) for k in ("xi", "eta", "xi_r", "eta_r",
                                                                               "Kt", "Kr", "U", "steps")})
                 self.t["sys"][a:b].fill_(-1)
@@ -246,6 +284,33 @@ class SlotArena(Arena):
         if done:
             self.lt["flags"][_lib.FLAG_REBUILD:_lib.FLAG_REBUILD + 1].fill_(1)
         return done
+
+    def slot_state(self, rec):
+        """Host copies (FP64) of the positions, velocities, orientations and angular momenta of the
+        system whose record is `rec`."""
+        a, b = int(rec["p_start"]), int(rec["p_end"])
+        g = lambda key: self.t[key][a:b].cpu().numpy()
+        return dict(x=np.stack([g("x"), g("y"), g("z")], 1), v=np.stack([g("vx"), g("vy"), g("vz")], 1),
+                    q=np.stack([g("q0"), g("q1"), g("q2"), g("q3")], 1),
+                    p=np.stack([g("p0"), g("p1"), g("p2"), g("p3")], 1))
+
+    def _deactivate(self, job: Job):
+        sp, _ = self.records()
+        rec = sp[job.slot].copy()
+        rec["active"] = 0
+        self._put_record("sp", job.slot, rec)
+
+    def cancel(self, job: Job):
+        """Stop a resident job at the current step (call between steps, stream idle); the next
+        harvest retires it with a 'cancelled' result."""
+        self._deactivate(job)
+        job.cancelled = True
+
+    def stop(self, job: Job, reason):
+        """Stop a resident job with an error (call between steps, stream idle); the next harvest
+        retires it with that error."""
+        self._deactivate(job)
+        job.error = reason
 
     def particle_steps(self):
         """Particle-steps done so far by the resident systems."""
@@ -304,15 +369,60 @@ class Server:
         self.job_bytes = {}
         self.chunk = chunk
         self.recoveries = 0
+        self._cancel = set()
+
+    def cancel(self, jid):
+        """Cancel a job by id: a waiting job is dropped, a resident one stops at the next chunk
+        boundary; either way it retires with an error result."""
+        self._cancel.add(jid)
 
     def _fits_budget(self, cr, job):
         b = cr.arena.job_bytes(job.spec)
         return self.resident_bytes + b <= self.mem_budget, b
 
-    def run(self, jobs, time_limit=None, log=None):
+    def _stop_collapsed(self, cr, err):
+        """A job whose system collapsed (a particle with more neighbours than the list build
+        sorts) is stopped with an error and the list rebuilt without it; the steps the overflow
+        skipped were skipped by every job of the class alike, so the others are unaffected."""
+        job = cr.arena.jobs.get(err.system)
+        if job is None:
+            raise err
+        st = cr.runner.stream
+        with torch.cuda.stream(st):
+            cr.arena.stop(job, f"the system collapsed: {err}")
+            cr.arena.build_list(True, st)
+        st.synchronize()
+        cr.arena.ensure_capacity()
+        cr.runner._capture()
+
+    def _apply_cancellations(self, queue, waiting, rejected):
+        if not self._cancel:
+            return
+        for dq in [queue] + list(waiting.values()):
+            for j in [j for j in dq if j.jid in self._cancel]:
+                dq.remove(j)
+                j.cancelled = True
+                j.result = {"error": "cancelled before admission"}
+                rejected.append(j)
+                self._cancel.discard(j.jid)
+        for cr in self.classes.values():
+            hit = [j for j in cr.arena.jobs.values() if j.jid in self._cancel]
+            if hit:
+                with torch.cuda.stream(cr.runner.stream):
+                    for j in hit:
+                        cr.arena.cancel(j)
+                        self._cancel.discard(j.jid)
+
+    def run(self, jobs, time_limit=None, log=None, on_chunk=None):
+        """Serve the jobs (arrival times relative to the start) until all are done or the time
+        limit passes.  on_chunk(server, seconds), if given, is called after every round of chunks
+        (streams idle): the place to cancel jobs or inspect the arenas.  A job whose state goes
+        non-finite is stopped by the device at that step and retired with an error result;
+        the other jobs are unaffected."""
         queue = collections.deque(sorted(jobs, key=lambda j: j.arrival))
         waiting = {name: collections.deque() for name in self.classes}
         finished = []
+        failed = []       # stopped on a non-finite state, or cancelled while resident
         rejected = []
         lat = []
         t0 = time.perf_counter()
@@ -325,6 +435,7 @@ class Server:
             while queue and queue[0].arrival <= now:
                 j = queue.popleft()
                 waiting[j.cls].append(j)
+            self._apply_cancellations(queue, waiting, rejected)
             # admission (FIFO per class; a job that does not fit blocks the ones behind it); all device
             # work of a class, copies included, stays on that class's stream
             for name, cr in self.classes.items():
@@ -356,7 +467,12 @@ class Server:
             evs = [(cr, cr.run_chunk(self.chunk)) for cr in busy]
             for cr, ev in evs:
                 cr.runner.stream.synchronize()
-                if cr.runner.recover():     # list overflow: those steps were skipped on the device
+                try:
+                    regrown = cr.runner.recover()   # list overflow: those steps were skipped on the device
+                except _lib.RowLimitError as e:
+                    self._stop_collapsed(cr, e)
+                    regrown = True
+                if regrown:
                     self.recoveries += 1
                     continue
                 lat.extend(ev[i].elapsed_time(ev[i + 1]) for i in range(len(ev) - 1))
@@ -364,9 +480,11 @@ class Server:
                     retired = cr.arena.harvest()
                 for j in retired:
                     j.done_at = time.perf_counter() - t0
-                    retired_ps += j.n * j.steps
+                    retired_ps += j.n * j.steps_done
                     self.resident_bytes -= self.job_bytes.pop(j.jid)
-                    finished.append(j)
+                    (failed if "error" in j.result else finished).append(j)
+            if on_chunk is not None:
+                on_chunk(self, now)
             if log and now - last_log > 5.0:
                 last_log = now
                 log(now, {n: len(c.arena.jobs) for n, c in self.classes.items()}, len(finished))
@@ -376,17 +494,20 @@ class Server:
         resident = []
         for cr in self.classes.values():
             if cr.arena.jobs:
-                _, th = cr.arena.records()
+                sp, th = cr.arena.records()
                 for slot, j in cr.arena.jobs.items():
                     j.steps_done = int(th[slot]["steps"])
+                    j.result = {"resident": True, **cr.arena.slot_state(sp[slot])}
                     resident.append(j)
         self.last_resident = resident
         for cr in self.classes.values():
             cr.arena.check()
         self.last_finished = finished
+        self.last_failed = failed
         self.last_rejected = rejected
         st = _stats("served", finished, retired_ps + resident_ps, wall, lat)
         st["jobs_rejected"] = len(rejected)
+        st["jobs_failed"] = len(failed)
         st["list_regrowths"] = self.recoveries
         st["mem_budget_bytes"] = self.mem_budget
         st["arena_bytes"] = self.arena_bytes
@@ -418,7 +539,7 @@ def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None)
         if j.arrival > now:
             time.sleep(j.arrival - now)
         shape, hyp, net = classes[j.cls]
-        ar = Arena([j.spec], shape, hyp, net, device=dev)
+        ar = Arena([_spec_of(j.spec)], shape, hyp, net, device=dev)
         runner = StepRunner(ar, use_graph=True)      # multi-step graphs for small systems
         runner.init()
         done_dev = 0
@@ -460,16 +581,28 @@ def run_plain(jobs, classes, time_limit=None, chunk=20, device="cuda", log=None)
     return st
 
 
-def plain_step_costs(jobs, classes, device="cuda", steps=20, warmup=5, log=None):
+def _reached_spec(job):
+    """The job's system at the state it reached (its result), else None."""
+    r = job.result
+    if not all(k in r for k in "xvqp"):
+        return None
+    s = _spec_of(job.spec)
+    return SystemSpec(r["x"], r["v"], r["q"], r["p"], s.mass, s.L, s.kT, s.dt, s.moi, s.rc, skin=s.skin, tau=s.tau)
+
+
+def plain_step_costs(jobs, classes, device="cuda", steps=20, warmup=5, log=None, reached=False):
     """Per job: ms per step of the plain single-system captured step on that job's system
-    (same N, box, kT), timed with hipEvents after `warmup` steps on a copy of the system."""
+    (same N, box, kT), timed with hipEvents after `warmup` steps on a copy of the system: its
+    initial state, or with `reached` the state the job reached (its result; jobs without one are
+    timed at the initial state).  Pair counts, and so step costs, change as a system evolves."""
     import copy
     dev = torch.device(device)
     costs = {}
     t0 = time.perf_counter()
     for k, j in enumerate(jobs):
         shape, hyp, net = classes[j.cls]
-        ar = Arena([copy.deepcopy(j.spec)], shape, hyp, net, device=dev)
+        spec = (_reached_spec(j) if reached else None) or copy.deepcopy(_spec_of(j.spec))
+        ar = Arena([spec], shape, hyp, net, device=dev)
         r = StepRunner(ar, use_graph=True)
         r.init()
         r.run_exact(warmup)
@@ -511,8 +644,8 @@ def same_work_plain(jobs, costs, sample_scale=1.0):
 
 def poisson_jobs(rate, duration, seed, shapes=("cube8", "tet10"), n_range=(128, 65536),
                  steps_range=(1000, 100000), kT_range=(0.5, 2.0), phi_range=(0.2, 0.45), max_jobs=None):
-    """Seeded job trace: Poisson arrivals, N and step budget log-uniform, kT and packing uniform."""
-    from .configs import make_system
+    """Seeded job trace: Poisson arrivals, N and step budget log-uniform, kT and packing uniform.
+    Each job's particle arrays are built when it is first looked at (LazySpec)."""
     from .shapes import get_shape
     rng = np.random.default_rng(seed)
     t, jobs = 0.0, []
@@ -528,6 +661,6 @@ def poisson_jobs(rate, duration, seed, shapes=("cube8", "tet10"), n_range=(128, 
         sh = get_shape(shape)
         # the minimum image needs a box of at least 2 rc: small systems get a lower packing fraction
         phi = min(phi, n * sh.volume / (2.05 * sh.cutoff) ** 3)
-        spec = make_system(shape, n, phi, kT=kT, seed=int(rng.integers(1 << 30)), non_overlap=False)
+        spec = LazySpec(shape, n, phi, kT=kT, seed=int(rng.integers(1 << 30)), non_overlap=False)
         jobs.append(Job(len(jobs), shape, spec, steps, t))
     return jobs

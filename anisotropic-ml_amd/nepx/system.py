@@ -199,9 +199,10 @@ class Arena:
         """Synchronising check after a build: grow the list if it overflowed or is nearly full."""
         f = self.flags()
         pl = int(f[_lib.FLAG_PL])
-        if f[_lib.FLAG_MAXROW] > 2048:
-            raise _lib.NepxError(f"a particle has {int(f[_lib.FLAG_MAXROW])} neighbours in the list "
-                                 "radius (maximum 2048)")
+        if f[_lib.FLAG_MAXROW] > _lib.MAX_ROW:
+            sysid = int(f[_lib.FLAG_LONGSYS]) - 1
+            raise _lib.RowLimitError(f"a particle of system {sysid} has {int(f[_lib.FLAG_MAXROW])} neighbours in "
+                                     f"the list radius (maximum {_lib.MAX_ROW})", sysid)
         if f[_lib.FLAG_OVERFLOW] or pl > 0.9 * self.cap:
             self._alloc_list(int(1.5 * max(pl, self.cap)) + 4096)
             self.build_list(True)
@@ -249,11 +250,24 @@ class Arena:
         """Steps system k has completed on the device (skipped steps are not counted)."""
         return int(self.thermo()[k]["steps"])
 
+    def faults(self):
+        """Systems whose state went non-finite; the device stopped each at that step."""
+        if not self.specs:
+            return []
+        th = self.thermo()
+        return [k for k in range(len(self.specs)) if th[k]["fault"] != 0]
+
     def check(self):
+        """Raise on a list overflow or on a system stopped by a non-finite state; returns the flags."""
         f = self.flags()
         if f[_lib.FLAG_OVERFLOW]:
             raise _lib.NepxError(f"neighbour list overflow (flags {f.tolist()}); results since the last "
                                  "check are invalid")
+        bad = self.faults()
+        if bad:
+            th = self.thermo()
+            raise _lib.NepxError("non-finite state: system(s) " +
+                                 ", ".join(f"{k} (stopped after {int(th[k]['steps'])} steps)" for k in bad))
         return f
 
     # ----------------------------------------------------------- components

@@ -209,9 +209,17 @@ def serve_bench(a):
     rng = np.random.default_rng(0)
     timed = (worked if len(worked) <= a.serve_plain_jobs else
              [worked[i] for i in rng.choice(len(worked), a.serve_plain_jobs, replace=False)])
-    costs = plain_step_costs(timed, classes, dev, log=log)
+    # each sampled job's plain step is timed at its initial state and at the state it reached in
+    # the served run (pair counts change as a system evolves); the comparison charges each job the
+    # mean of the two costs (trapezoid over its run) and reports both ends
+    c0 = plain_step_costs(timed, classes, dev, log=log)
+    c1 = plain_step_costs(timed, classes, dev, log=log, reached=True)
+    costs = {k: 0.5 * (c0[k] + c1[k]) for k in c0}
     # uniform sample: scale the sampled plain time up to all jobs
-    plain_same = same_work_plain(worked, costs, sample_scale=len(worked) / max(len(timed), 1))
+    scale = len(worked) / max(len(timed), 1)
+    plain_same = same_work_plain(worked, costs, sample_scale=scale)
+    plain_initial = same_work_plain(worked, c0, sample_scale=scale)
+    plain_reached = same_work_plain(worked, c1, sample_scale=scale)
     res = {
         "metric": "served-batch vs plain particle-steps/s (mixed cube/tet/cylinder job trace); p99 step latency",
         "value": served["particle_steps_per_s"],
@@ -221,6 +229,8 @@ def serve_bench(a):
         "served_over_plain": served["particle_steps_per_s"] / plain_same["particle_steps_per_s"],
         "served": served,
         "plain_same_work": plain_same,
+        "plain_same_work_initial_states": plain_initial,
+        "plain_same_work_reached_states": plain_reached,
         "config": {"workload": f"Poisson 20 jobs/s for {a.serve_arrivals} s of arrivals ({len(jobs)} jobs), "
                                f"{'/'.join(classes)}, N log-uniform [128, {a.serve_n_max}], steps log-uniform [1e3, 1e5], "
                                "kT U[0.5, 2], phi U[0.2, 0.45], FFNN D->64->64->1 (D=40) or D->128->128->1 "

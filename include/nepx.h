@@ -69,7 +69,7 @@ typedef struct nepx_sysparam {
     int32_t blk_start, blk_end;          /* integrator block range               */
     int32_t nc, cell_off;                /* cells per side (0 = all pairs), offset */
     int32_t active;                      /* 0: the system is skipped by every kernel */
-    int32_t ipad[9];                     /* zero; ipad[0] is the integrator's per-step block count */
+    int32_t ipad[9];                     /* zero */
 } nepx_sysparam;
 
 /* Thermostat record per system (Sim.py:99-111, IntegrateStepTwoKernels.py:74-130). */
@@ -80,7 +80,9 @@ typedef struct nepx_thermo {
     double max_disp2;                /* largest squared displacement since build */
     double steps, budget;            /* steps done; the system deactivates itself at budget (> 0) */
     double pending;                  /* != 0: initial kinetic energies + first update still to do */
-    double pad[3];
+    double fault;                    /* 1: energies or displacement went non-finite; the system was
+                                        deactivated (active = 0) at that step, its steps not counted */
+    double pad[2];
 } nepx_thermo;
 
 /* Particle arena, SoA, device pointers (Sim.py:147-169 holds the same state AoS f32). */
@@ -120,9 +122,11 @@ typedef struct nepx_nlist {
     int32_t* rnbr;                   /* cap: reverse rows, then slot ids          */
     int32_t* cidx;                   /* cap: compacted index of slot or -1        */
     int32_t* blk_count;              /* compaction partials (cap/1024 rounded up)  */
-    int32_t* flags;                  /* [0] list stale (rebuild; steps wait), [1] overflow, [2] P_L,
-                                        [3] P, [4] builds, [5] longest row, [6] step frozen (set by the
-                                        filter while [0] is set), [7] rebuild at the next build point */
+    int32_t* flags;                  /* 16 ints: [0] list stale (rebuild; steps wait), [1] overflow,
+                                        [2] P_L, [3] P, [4] builds, [5] longest row (when over 2048),
+                                        [6] step frozen (set by the filter while [0] is set), [7] rebuild
+                                        at the next build point, [8] 1 + a system with a row over 2048
+                                        (0: none), others zero */
     int32_t* pair_i;                 /* cap: compacted pairs                       */
     int32_t* pair_j;
     float* pair_t;                   /* cap*4: minimum-image t = X_j - X_i (f32)  */

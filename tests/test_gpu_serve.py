@@ -94,3 +94,71 @@ def test_admission_waits_for_memory_budget(gpu):
     st = srv.run([a, b])
     assert st["jobs_finished"] == 2 and st["peak_resident_job_bytes"] <= int(1.5 * one)
     assert b.admitted_at > 0 and b.admitted_at >= a.done_at
+
+
+def test_non_finite_job_and_cancelled_job_retire_alone(gpu):
+    """Mid-run, one job's velocities are poisoned with a NaN and another job is cancelled: the first
+    is stopped by the device at that step and retires with a fault, the second retires as cancelled,
+    and the remaining jobs finish bitwise where they finish when served alone."""
+    from nepx.configs import make_system
+    from nepx.serve import Job, Server
+    classes = _classes(gpu)
+
+    def jobs():
+        return [Job(0, "tet10", make_system("tet10", 300, 0.1, kT=0.8, seed=31), 25, 0.0),
+                Job(1, "tet10", make_system("tet10", 400, 0.2, kT=1.2, seed=32), 25, 0.0),
+                Job(2, "cube8", make_system("cube8", 216, 0.25, kT=1.0, seed=33, non_overlap=False), 25, 0.0),
+                Job(3, "tet10", make_system("tet10", 350, 0.15, kT=1.0, seed=34), 400, 0.0)]
+    js = jobs()
+    calls = []
+
+    def hook(server, now):
+        calls.append(now)
+        if len(calls) == 1:  # after the first chunk (5 steps)
+            ar = server.classes["tet10"].arena
+            a = int(ar.records()[0][js[1].slot]["p_start"])
+            ar.t["vx"][a + 7] = float("nan")
+            torch.cuda.synchronize()
+            server.cancel(3)
+
+    srv = Server(classes, capacity=2048, max_systems=4, chunk=5, device=gpu)
+    st = srv.run(js, on_chunk=hook)
+    for j in js:
+        print(f"job {j.jid} steps_done={j.steps_done} {j.result.get('error', '')}")
+    assert st["jobs_finished"] == 2 and st["jobs_failed"] == 2
+    assert js[1].result.get("fault") and js[1].steps_done == 5
+    assert "cancelled" in js[3].result["error"] and 5 <= js[3].steps_done < 400
+    for j in (js[0], js[2]):
+        ref = jobs()[j.jid]
+        Server(classes, capacity=2048, max_systems=4, chunk=5, device=gpu).run([ref])
+        for key in "xvqp":
+            assert np.array_equal(ref.result[key], j.result[key]), (j.jid, key)
+
+
+def test_collapsed_job_is_stopped_and_the_rest_continue(gpu):
+    """A job whose 2,100 bodies sit in one small ball (every particle has more neighbours in the
+    list radius than the build sorts) is stopped with an error at its first list build; the steps
+    the overflow skipped were skipped by every job alike, and the others finish bitwise where they
+    finish alone."""
+    from nepx.configs import make_system
+    from nepx.serve import Job, Server
+    classes = _classes(gpu)
+    blob = make_system("tet10", 2100, 0.05, seed=35)
+    blob.x = np.random.default_rng(3).uniform(-1.0, 1.0, (2100, 3))
+
+    def jobs():
+        return [Job(0, "tet10", make_system("tet10", 300, 0.1, kT=0.8, seed=31), 20, 0.0),
+                Job(1, "tet10", blob, 20, 0.0),
+                Job(2, "cube8", make_system("cube8", 216, 0.25, kT=1.0, seed=33, non_overlap=False), 20, 0.0)]
+    js = jobs()
+    srv = Server(classes, capacity=4096, max_systems=4, chunk=5, device=gpu)
+    st = srv.run(js)
+    for j in js:
+        print(f"job {j.jid} steps_done={j.steps_done} {j.result.get('error', '')}")
+    assert st["jobs_finished"] == 2 and st["jobs_failed"] == 1
+    assert "collapsed" in js[1].result["error"] and js[1].steps_done == 0
+    for j in (js[0], js[2]):
+        ref = jobs()[j.jid]
+        Server(classes, capacity=4096, max_systems=4, chunk=5, device=gpu).run([ref])
+        for key in "xvqp":
+            assert np.array_equal(ref.result[key], j.result[key]), (j.jid, key)

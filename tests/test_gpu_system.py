@@ -170,3 +170,40 @@ def test_list_overflow_skips_steps_and_recovers(gpu):
     a, b = ref.system_arrays(0), ar.system_arrays(0)
     for k in ("x", "v", "q", "p", "F", "T"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_non_finite_state_stops_only_its_system(gpu):
+    """A NaN put into one system's velocities: the device stops that system at the next step (its
+    step count stays, check() names it) while the other system of the batch runs on and ends
+    bitwise where it ends when run alone."""
+    import nepx
+    from nepx import _lib
+    from nepx.configs import make_system
+    from nepx.engine import StepRunner
+    net = nepx.EnergyNet.random(40, [64, 64], seed=3, device=gpu)
+    shape = nepx.get_shape("tet10")
+    mk = lambda: [make_system("tet10", 600, 0.3, seed=41), make_system("tet10", 500, 0.3, kT=1.3, seed=42)]
+    alone = nepx.Arena([mk()[1]], shape, (11, 6, 4), net, device=gpu)
+    ra = StepRunner(alone, use_graph=True, steps_per_graph=1)
+    ra.init()
+    ra.run(12)
+    torch.cuda.synchronize()
+    alone.check()
+    ar = nepx.Arena(mk(), shape, (11, 6, 4), net, device=gpu)
+    r = StepRunner(ar, use_graph=True)
+    r.init()
+    r.run(4)
+    torch.cuda.synchronize()
+    ar.check()
+    ar.t["vx"][ar.starts[0] + 17] = float("nan")
+    torch.cuda.synchronize()
+    r.run(8)
+    torch.cuda.synchronize()
+    with pytest.raises(_lib.NepxError, match="non-finite"):
+        ar.check()
+    th = ar.thermo()
+    assert ar.faults() == [0] and int(th[0]["steps"]) == 4 and int(th[1]["steps"]) == 12
+    assert ar.t["sp"].cpu().numpy().view(_lib.SYSPARAM_DTYPE)[0]["active"] == 0
+    a, b = alone.system_arrays(0), ar.system_arrays(1)
+    for k in ("x", "v", "q", "p", "F", "T"):
+        assert np.array_equal(a[k], b[k]), k
