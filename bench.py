@@ -383,8 +383,16 @@ def long_run_bench(a):
     rows = []
     t0 = time.perf_counter()
     overflows = 0
+    stopped = None
     for c in range(n_chunks):
-        overflows += run_recovering(runner, chunk)
+        try:
+            overflows += run_recovering(runner, chunk)
+        except _lib.RowLimitError as e:
+            # the random-init network is attractive at short range, so the system condenses; the list
+            # build reports a particle with more neighbours than it sorts and the run ends there
+            stopped = {"after_step": int(ar.device_steps()), "reason": str(e)}
+            print(f"[long] stopped: {e}", file=sys.stderr, flush=True)
+            break
         th = ar.thermo()[0]
         rec = {k: float(th[k]) for k in ("xi", "eta", "xi_r", "eta_r", "Kt", "Kr", "U", "steps")}
         if not all(np.isfinite(v) for v in rec.values()):
@@ -395,6 +403,9 @@ def long_run_bench(a):
             print(f"[long] step {(c + 1) * chunk}: T_t {2 * rec['Kt'] / spec.ndof:.4f} "
                   f"T_r {2 * rec['Kr'] / spec.rndof:.4f} U {rec['U']:.1f}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    if len(rows) < 2:
+        print(json.dumps({"error": "the run stopped before two samples", "stopped": stopped}))
+        sys.exit(4)
     x = ar.system_arrays(0)
     if not all(np.all(np.isfinite(x[k])) for k in ("x", "v", "q", "p")):
         print(json.dumps({"error": "non-finite particle state at the end"}))
@@ -424,13 +435,16 @@ def long_run_bench(a):
                                     "energies do not vanish at the COM cutoff, so pairs crossing it change U "
                                     "discontinuously"},
         "samples": len(rows), "sample_every": chunk, "wall_s": wall, "list_overflows_recovered": overflows,
+        "stopped": stopped,
+        "U": {"first": float(rows[0]["U"]), "last": float(rows[-1]["U"])},
         "config": {"workload": f"{w['n']} {w['shape']} phi={w['phi']} rc={shape.cutoff} hypers={hyp} D=40 "
                                f"FFNN 40->64->64->1 tanh ({net.precision}), NVT kT={kT} dt={w['dt']} tau={tau}",
                    "particles": spec.n, "list_builds": int(ar.flags()[_lib.FLAG_BUILDS])},
         "series": {"step": steps[:: max(1, len(rows) // 20)].tolist(),
                    "T_trans": Tt[:: max(1, len(rows) // 20)].tolist(),
                    "T_rot": Tr[:: max(1, len(rows) // 20)].tolist(),
-                   "H": H[:: max(1, len(rows) // 20)].tolist()},
+                   "H": H[:: max(1, len(rows) // 20)].tolist(),
+                   "U": [r["U"] for r in rows][:: max(1, len(rows) // 20)]},
         "data": "synthetic",
     }
 

@@ -102,3 +102,32 @@ def test_bead_nvt_extended_energy_is_second_order(gpu):
           f"ratio {e1 / e2:.2f}")
     assert umax - umin > 100.0
     assert e1 < 3e-3 * h0 and 3.3 < e1 / e2 < 5.2
+
+
+def test_bead_nvt_temperatures_reach_kT(gpu):
+    """512 cubes under the smooth bead potential, started at kT 1.5 and coupled to kT 0.8
+    (tau 0.1): over the second half of 3,200 captured steps the translational and rotational
+    temperatures 2 K_t / N_dof and 2 K_r / RN_dof average kT within 5 %."""
+    import nepx
+    from nepx import label
+    from nepx.configs import make_system
+    from nepx.engine import StepRunner
+    pot = label.BeadPotential(device=gpu)
+    spec = make_system("cube8", 512, 0.004, kT=1.5, dt=0.002, seed=21, rc=pot.reach, non_overlap=False)
+    spec.kT, spec.tau = 0.8, 0.1
+    ar = nepx.Arena([spec], nepx.get_shape("cube8"), (11, 6, 4), None, device=gpu, clip=0.0, potential=pot)
+    r = StepRunner(ar, use_graph=True)
+    r.init()
+    Tt, Tr = [], []
+    th = ar.thermo()[0]
+    t0 = (2 * th["Kt"] / spec.ndof, 2 * th["Kr"] / spec.rndof)
+    for _ in range(200):
+        r.run_exact(16)
+        th = ar.thermo()[0]
+        Tt.append(2 * th["Kt"] / spec.ndof)
+        Tr.append(2 * th["Kr"] / spec.rndof)
+    ar.check()
+    mt, mr = float(np.mean(Tt[100:])) / spec.kT, float(np.mean(Tr[100:])) / spec.kT
+    print(f"bead NVT: T_t {t0[0]:.3f} -> <T_t>/kT {mt:.4f}, T_r {t0[1]:.3f} -> <T_r>/kT {mr:.4f} "
+          f"(std {np.std(Tt[100:]) / spec.kT:.3f}, {np.std(Tr[100:]) / spec.kT:.3f})")
+    assert t0[0] > 1.2 and abs(mt - 1.0) < 0.05 and abs(mr - 1.0) < 0.05
