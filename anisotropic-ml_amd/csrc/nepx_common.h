@@ -50,6 +50,10 @@ constexpr double cfact(int n) {
     return r;
 }
 
+// ---- list flags (nepx_nlist.flags) -----------------------------------------------
+enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5, F_FROZEN = 6, F_SOON = 7,
+       F_LONGSYS = 8, F_TAG = 9, F_RTAG = 10 };
+
 // ---- XCD-contiguous block order ---------------------------------------------
 // The dispatcher deals a grid's blocks to the 8 XCDs in turn (block b to XCD b % 8),
 // and each XCD has its own L2.  Kernels whose neighbouring blocks share data take

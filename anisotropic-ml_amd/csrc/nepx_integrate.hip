@@ -420,8 +420,7 @@ extern "C" int nepx_md_step(nepx_state* st, nepx_nlist* nl, const nepx_pair_mode
     if (rc) return rc;
     rc = nepx_pairs_filter(st, nl, stream);
     if (rc) return rc;
-    rc = nepx_pair_fused(m, net, st->qf, nl->pair_i, nl->pair_j, nl->pair_t, nl->flags + 3, nl->cap, clip,
-                         pair_out, stream);
+    rc = nepx_pair_fused_list(m, net, st, nl, clip, pair_out, stream);
     if (rc) return rc;
     rc = nepx_forces_gather(st, nl, pair_out, stream);
     if (rc) return rc;

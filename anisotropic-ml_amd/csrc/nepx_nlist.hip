@@ -21,9 +21,6 @@ constexpr int kScanBlock = 1024;
 constexpr int kSortMax = 2048;  // longest row the in-LDS sort accepts
 constexpr int kSlotsPerBlock = 1024;
 
-enum { F_REBUILD = 0, F_OVERFLOW = 1, F_PL = 2, F_P = 3, F_BUILDS = 4, F_MAXROW = 5, F_FROZEN = 6, F_SOON = 7,
-       F_LONGSYS = 8 };
-
 __device__ __forceinline__ double wrap1(double t, double L) {
     t = t > 0.5 * L ? t - L : t;
     t = t < -0.5 * L ? L + t : t;
@@ -340,6 +337,7 @@ __global__ __launch_bounds__(kNB) void k_rev_slots(nepx_nlist nl, int n) {
         }
         if (slot < 0) nl.flags[F_OVERFLOW] = 2;  // inconsistent list: never expected
         nl.rnbr[e] = slot;
+        if (nl.rpos && slot >= 0) nl.rpos[slot] = e;  // where the reverse row of k holds this slot
     }
 }
 
@@ -484,6 +482,8 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
         if (blockIdx.x == 0 && threadIdx.x == 0) nl.flags[F_P] = 0;
         return;
     }
+    // a new pass: the pair kernels tag the reverse-order records they write with it
+    if (blockIdx.x == 0 && threadIdx.x == 0) nl.flags[F_TAG] = nl.flags[F_TAG] + 1;
     if (base >= PL) return;
     const int first = block_prefix(nl.blk_count, blockIdx.x);
     if (base + kSlotsPerBlock >= PL && threadIdx.x == 0) nl.flags[F_P] = first + nl.blk_count[blockIdx.x];
@@ -515,6 +515,7 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
                 nl.cidx[s] = c;
                 nl.pair_i[c] = r.i[k];
                 nl.pair_j[c] = r.j[k];
+                if (nl.pair_e) nl.pair_e[c] = nl.rpos[s];
                 reinterpret_cast<float4*>(nl.pair_t)[c] =
                     make_float4((float)r.tx[k], (float)r.ty[k], (float)r.tz[k], 0.f);
             } else {
@@ -529,7 +530,10 @@ __global__ __launch_bounds__(kNB) void k_filter_write(nepx_state st, nepx_nlist 
 // GL lanes per particle (tuned; 16 by default): lanes stride over the
 // particle's forward row (i-role: +F, +tau_i, +E) and reverse row (j-role:
 // -F, +tau_j), accumulate in FP64 and reduce with a fixed butterfly, so the
-// result is independent of timing (and, for one GL, of the launch).
+// result is independent of timing (and, for one GL, of the launch).  The j side
+// comes from the reverse-order records when this pass's pair kernel wrote them
+// (the same values, so the same sums bit for bit), else through the slot ->
+// compacted-index map from the per-pair records.
 template <int GL>
 __device__ __forceinline__ double red_lanes(double v) {
 #pragma unroll
@@ -545,6 +549,9 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
     const int sub = threadIdx.x % GL;
     if (k >= st.n) return;  // whole lane groups exit together
     if (nl.flags[F_OVERFLOW] || nl.flags[F_FROZEN]) return;  // skipped step: forces stay as they are
+    // the pair kernel of this pass wrote reverse-order records (uniform: every thread reads the same flags)
+    const int tag = nl.flags[F_TAG];
+    const bool rev = nl.rrec != nullptr && nl.flags[F_RTAG] == tag;
     double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, u = 0;
     if (live(st, k)) {
         for (int s = nl.row_start[k] + sub; s < nl.row_start[k + 1]; s += GL) {
@@ -559,16 +566,31 @@ __global__ __launch_bounds__(kNB) void k_gather(nepx_state st, nepx_nlist nl, co
             ty += b.y;
             tz += b.z;
         }
-        for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += GL) {
-            const int c = nl.cidx[nl.rnbr[e]];
-            if (c < 0) continue;
-            const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 2];
-            fx -= a.x;
-            fy -= a.y;
-            fz -= a.z;
-            tx += b.x;
-            ty += b.y;
-            tz += b.z;
+        if (rev) {  // this pass's j-side records, in reverse-row order: read contiguously
+            const float4* rr = reinterpret_cast<const float4*>(nl.rrec);
+            for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += GL) {
+                const float4 a = rr[2 * (size_t)e];
+                if (__float_as_int(a.w) != tag) continue;  // no active pair at this entry this step
+                const float4 b = rr[2 * (size_t)e + 1];
+                fx -= a.x;
+                fy -= a.y;
+                fz -= a.z;
+                tx += b.x;
+                ty += b.y;
+                tz += b.z;
+            }
+        } else {
+            for (int e = nl.rrow_start[k] + sub; e < nl.rrow_start[k + 1]; e += GL) {
+                const int c = nl.cidx[nl.rnbr[e]];
+                if (c < 0) continue;
+                const float4 a = out[3 * (size_t)c], b = out[3 * (size_t)c + 2];
+                fx -= a.x;
+                fy -= a.y;
+                fz -= a.z;
+                tx += b.x;
+                ty += b.y;
+                tz += b.z;
+            }
         }
     }
     fx = red_lanes<GL>(fx);
@@ -600,6 +622,8 @@ static int check_state_core(const nepx_state* st, const nepx_nlist* nl) {
                  NEPX_EINVAL, "null list buffer");
     NEPX_REQUIRE(nl->cap > 0 && nl->cap < (int64_t)1 << 31, NEPX_ECAP, "list capacity %lld out of range",
                  (long long)nl->cap);
+    NEPX_REQUIRE(!(nl->pair_e || nl->rrec) || (nl->rpos && nl->pair_e && nl->rrec), NEPX_EINVAL,
+                 "reverse-order records need rpos, pair_e and rrec together");
     return NEPX_OK;
 }
 

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
                                                                 const int32_t* __restrict__ pj,
                                                                 const float4* __restrict__ pt,
                                                                 const int32_t* __restrict__ np_dev, int64_t maxp,
-                                                                float clip, float* __restrict__ out) {
+                                                                float clip, float* __restrict__ out, RevOut rv) {
     constexpr int LD = (Cfg::D + 3) / 4 * 4 + 4;  // row stride in LDS (16-byte aligned)
     __shared__ __attribute__((aligned(16))) float tile[kBlock * LD];
     __shared__ float etile[kBlock];
@@ -143,6 +143,7 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
     const void* WH;
     stage_net<N, PREC>(wl, Wg, Whg, W, WH);
     const int P = n_pairs(np_dev, maxp);
+    const int tag = rev_tag(rv);
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     float* myrow = tile + threadIdx.x * LD;
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_fused(nepx_pair_model 
                     desc_adjoint<Cfg, LR, decltype(first)::value>(m, g, dG, Ap, acc);
                 }
             });
-            store_out(out, idx, adj_finish<Cfg>(acc), etile[threadIdx.x], clip);
+            store_out(out, idx, adj_finish<Cfg>(acc), etile[threadIdx.x], clip, rv, tag);
         }
         __syncthreads();
     }
@@ -621,11 +622,11 @@ extern "C" int nepx_pair_fused_supported(const nepx_pair_model* m, const nepx_ne
 template <class Cfg, class N, int PREC>
 static void launch_fused_p(const nepx_pair_model& m, const nepx_net* net, int grid, float scale, const float* qf,
                            const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
-                           const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+                           const int32_t* np_dev, int64_t maxp, float clip, float* out, RevOut rv, void* stream) {
 #define NEPX_LAUNCH_FUSED(A)                                                                                     \
     hipLaunchKernelGGL((k_fused<Cfg, N, PREC, A>), dim3(grid), dim3(kBlock), 0, as_stream(stream), m,            \
                        net->w_packed, net->w_half, net->en_min, scale, (const float4*)qf, pair_i, pair_j,          \
-                       (const float4*)pair_t, np_dev, maxp, clip, out)
+                       (const float4*)pair_t, np_dev, maxp, clip, out, rv)
     if (net->act == NEPX_ACT_TANH) NEPX_LAUNCH_FUSED(NEPX_ACT_TANH);
     else if constexpr (PREC == NEPX_NET_F32 || PREC == NEPX_NET_F16X3) NEPX_LAUNCH_FUSED(NEPX_ACT_RELU);
 #undef NEPX_LAUNCH_FUSED
@@ -634,39 +635,39 @@ static void launch_fused_p(const nepx_pair_model& m, const nepx_net* net, int gr
 template <class Cfg, class N>
 static int launch_fused(const nepx_pair_model& m, const nepx_net* net, float scale,
                         const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
-                        const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+                        const int32_t* np_dev, int64_t maxp, float clip, float* out, RevOut rv, void* stream) {
     const int64_t cap = (int64_t)device_cus() * tuning().fused_blocks_per_cu;
     // four lanes per pair (site sums on MFMA) where the shape allows it, unless tuned off
     if (tuning().pair_lanes == 4) {
         const int rc = launch_fused_quad<Cfg, N>(m, net, net->precision, (int)cap, scale, qf, pair_i, pair_j,
-                                                 pair_t, np_dev, maxp, clip, out, stream);
+                                                 pair_t, np_dev, maxp, clip, out, rv, stream);
         if (rc != NEPX_EUNSUP) return rc;
     }
     const int grid = grid_for_cap(maxp, kBlock, cap);
     switch (net->precision) {
         case NEPX_NET_F16X3:
             launch_fused_p<Cfg, N, NEPX_NET_F16X3>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp,
-                                                    clip, out, stream);
+                                                    clip, out, rv, stream);
             break;
         case NEPX_NET_BF16:
             launch_fused_p<Cfg, N, NEPX_NET_BF16>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp,
-                                                   clip, out, stream);
+                                                   clip, out, rv, stream);
             break;
         case NEPX_NET_BF16X3:
             launch_fused_p<Cfg, N, NEPX_NET_BF16X3>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev,
-                                                     maxp, clip, out, stream);
+                                                     maxp, clip, out, rv, stream);
             break;
         default:
             launch_fused_p<Cfg, N, NEPX_NET_F32>(m, net, grid, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp,
-                                                  clip, out, stream);
+                                                  clip, out, rv, stream);
     }
     NEPX_LAUNCH_CHECK();
     return NEPX_OK;
 }
 
-extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
-                               const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
-                               const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+static int pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf, const int32_t* pair_i,
+                      const int32_t* pair_j, const float* pair_t, const int32_t* np_dev, int64_t maxp, float clip,
+                      float* out, RevOut rv, void* stream) {
     int rc = check_model(m);
     if (rc) return rc;
     rc = check_net(net, model_D(m));
@@ -679,12 +680,33 @@ extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, co
 #define X(ns, a, b, c, T, d, w, h, NT)                                                               \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c && net->d_in == d &&        \
         net->width == w && net->n_hidden == h)                                                       \
-        return launch_fused<T, NT>(*m, net, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip, out, stream);
+        return launch_fused<T, NT>(*m, net, scale, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip, out, rv, stream);
     NEPX_FUSED_CASES(X)
 #undef X
     set_error("no fused kernel for n_sites=%d hypers=(%d,%d,%d) net %d->%dx%d", m->n_sites, m->nrad,
               m->nang, m->lmax, net->d_in, net->width, net->n_hidden + 1);
     return NEPX_EUNSUP;
+}
+
+extern "C" int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* qf,
+                               const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
+                               const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream) {
+    return pair_fused(m, net, qf, pair_i, pair_j, pair_t, np_dev, maxp, clip, out, RevOut{}, stream);
+}
+
+extern "C" int nepx_pair_fused_list(const nepx_pair_model* m, const nepx_net* net, const nepx_state* st,
+                                    nepx_nlist* nl, float clip, float* out, void* stream) {
+    NEPX_REQUIRE(st && nl && st->qf && nl->flags, NEPX_EINVAL, "null state or list");
+    NEPX_REQUIRE(!(nl->pair_e || nl->rrec) || (nl->rpos && nl->pair_e && nl->rrec), NEPX_EINVAL,
+                 "reverse-order records need rpos, pair_e and rrec together");
+    RevOut rv;
+    if (nl->rrec) {
+        rv.pe = nl->pair_e;
+        rv.rrec = nl->rrec;
+        rv.flags = nl->flags;
+    }
+    return pair_fused(m, net, st->qf, nl->pair_i, nl->pair_j, nl->pair_t, nl->flags + F_P, nl->cap, clip, out, rv,
+                      stream);
 }
 
 extern "C" int nepx_stencil_rows(const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,

@@ -60,12 +60,36 @@ __device__ __forceinline__ void load_row(const float* __restrict__ src, float (&
     }
 }
 
+// Optional second output of the fused pair kernels: the j side of pair idx, {F_i, tag}{tau_j, 0},
+// also goes to rrec at the pair's reverse-row entry pe[idx], so that the gather reads it
+// contiguously along each particle's reverse row.  tag = flags[F_TAG] of this pass, which the
+// kernel records in flags[F_RTAG].  All null: not written.
+struct RevOut {
+    const int32_t* pe = nullptr;
+    float* rrec = nullptr;
+    int32_t* flags = nullptr;
+};
+
+__device__ __forceinline__ int rev_tag(const RevOut& rv) {
+    if (!rv.flags) return 0;
+    const int tag = rv.flags[F_TAG];
+    if (blockIdx.x == 0 && threadIdx.x == 0) rv.flags[F_RTAG] = tag;
+    return tag;
+}
+
 __device__ __forceinline__ void store_out(float* __restrict__ out, int idx, const PairOut& o, float E,
-                                          float clip) {
+                                          float clip, const RevOut& rv = RevOut{}, int tag = 0) {
     float4* o4 = reinterpret_cast<float4*>(out) + 3 * (size_t)idx;
-    o4[0] = make_float4(clipf(o.fx, clip), clipf(o.fy, clip), clipf(o.fz, clip), E);
+    const float4 f = make_float4(clipf(o.fx, clip), clipf(o.fy, clip), clipf(o.fz, clip), E);
+    const float4 tj = make_float4(clipf(o.tjx, clip), clipf(o.tjy, clip), clipf(o.tjz, clip), 0.f);
+    o4[0] = f;
     o4[1] = make_float4(clipf(o.tix, clip), clipf(o.tiy, clip), clipf(o.tiz, clip), 0.f);
-    o4[2] = make_float4(clipf(o.tjx, clip), clipf(o.tjy, clip), clipf(o.tjz, clip), 0.f);
+    o4[2] = tj;
+    if (rv.rrec) {
+        float4* r4 = reinterpret_cast<float4*>(rv.rrec) + 2 * (size_t)rv.pe[idx];
+        r4[0] = make_float4(f.x, f.y, f.z, __int_as_float(tag));
+        r4[1] = tj;
+    }
 }
 
 // Input chunks of one pair for the MFMA tile: chunk c holds inputs
@@ -185,6 +209,6 @@ struct QuadCfg {
 template <class Cfg, class N>
 int launch_fused_quad(const nepx_pair_model& m, const nepx_net* net, int prec, int max_blocks, float scale,
                       const float* qf, const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
-                      const int32_t* np_dev, int64_t maxp, float clip, float* out, void* stream);
+                      const int32_t* np_dev, int64_t maxp, float clip, float* out, RevOut rv, void* stream);
 
 }  // namespace nepx

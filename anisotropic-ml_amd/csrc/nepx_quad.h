@@ -78,7 +78,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     nepx_pair_model m, const float* __restrict__ Wg, const void* __restrict__ Whg, float en_min, float scale,
     const float4* __restrict__ qf, const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
     const float4* __restrict__ pt, const int32_t* __restrict__ np_dev, int64_t maxp, float clip,
-    float* __restrict__ out) {
+    float* __restrict__ out, RevOut rv) {
     using Q = QuadCfg<Cfg>;
     static_assert(Q::OK, "shape not supported by the four-lane kernel");
     constexpr int NC = Q::NC, LC = Q::LC, NA = Cfg::NANG + 1, NLM = Cfg::NLM, LMAX = Cfg::LMAX;
@@ -104,6 +104,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     __syncthreads();
 
     const int P = n_pairs(np_dev, maxp);
+    const int tag = rev_tag(rv);
     const int lane = threadIdx.x & 63, q = lane & 3, wv = threadIdx.x >> 6, slot = lane >> 2;
     float* const xw = xch[wv];
     float* const xp = xw + slot * Q::TS;  // this pair's exchange area
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
         acc.G0x = quad_sum(ig_x); acc.G0y = quad_sum(ig_y); acc.G0z = quad_sum(ig_z);
         acc.T0x = quad_sum(it_x); acc.T0y = quad_sum(it_y); acc.T0z = quad_sum(it_z);
         acc.A0x = g.six; acc.A0y = g.siy; acc.A0z = g.siz;
-        if (valid && q == 0) store_out(out, idx, adj_finish<Cfg>(acc), E, clip);
+        if (valid && q == 0) store_out(out, idx, adj_finish<Cfg>(acc), E, clip, rv, tag);
         wave_lds_order();  // the row reads are done before the next iteration's exchange writes
     }
 }

@@ -48,7 +48,7 @@ class State(C.Structure):
 
 _NL_PTRS = ["cell_count", "cell_start", "cell_fill", "cell_list", "cell_pos", "cell_of", "row_count", "row_start",
             "rrow_count", "rrow_start", "nbr", "slot_row", "rnbr", "cidx", "blk_count", "flags", "pair_i",
-            "pair_j", "pair_t"]
+            "pair_j", "pair_t", "rpos", "pair_e", "rrec"]
 
 
 class NList(C.Structure):
@@ -71,7 +71,8 @@ THERMO_DTYPE = np.dtype([("xi", "f8"), ("eta", "f8"), ("xi_r", "f8"), ("eta_r", 
                          ("steps", "f8"), ("budget", "f8"), ("pending", "f8"), ("fault", "f8"),
                          ("pad", "f8", 2)])
 
-FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW, FLAG_FROZEN, FLAG_SOON, FLAG_LONGSYS = range(9)
+(FLAG_REBUILD, FLAG_OVERFLOW, FLAG_PL, FLAG_P, FLAG_BUILDS, FLAG_MAXROW, FLAG_FROZEN, FLAG_SOON, FLAG_LONGSYS,
+ FLAG_TAG, FLAG_RTAG) = range(11)
 MAX_ROW = 2048       # longest neighbour row the list build sorts
 NVT_STEP_TWO, NVT_TRANSLATE, NVT_ROTATE, NVT_THERMO = 2, 3, 4, 5
 N_FLAGS = 16
@@ -115,6 +116,8 @@ def _load():
                                      vp, vp], C.c_int),
         "nepx_pair_fused": ([C.POINTER(PairModel), C.POINTER(Net), vp, vp, vp, vp, vp, C.c_int64, C.c_float,
                              vp, vp], C.c_int),
+        "nepx_pair_fused_list": ([C.POINTER(PairModel), C.POINTER(Net), C.POINTER(State), C.POINTER(NList),
+                                  C.c_float, vp, vp], C.c_int),
         "nepx_pair_fused_supported": ([C.POINTER(PairModel), C.POINTER(Net)], C.c_int),
         "nepx_bead_pair_forces": ([vp, C.c_int32, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp, vp, C.c_int64,
                                    C.c_float, vp, vp], C.c_int),

@@ -136,7 +136,8 @@ class SlotArena(Arena):
                                    for k in ("cell_list", "cell_of", "cell_pos", "row_count", "row_start",
                                              "rrow_count", "rrow_start"))) / n
         self.slot_bytes = (sum(self.lt[k].numel() * self.lt[k].element_size()
-                               for k in ("nbr", "slot_row", "rnbr", "cidx", "pair_i", "pair_j", "pair_t")) +
+                               for k in ("nbr", "slot_row", "rnbr", "cidx", "pair_i", "pair_j", "pair_t", "rpos",
+                                         "pair_e", "rrec") if k in self.lt) +
                            self.pair_out.numel() * self.pair_out.element_size()) / cap
         self.cell_bytes = sum(self.lt[k].numel() * self.lt[k].element_size()
                               for k in ("cell_count", "cell_start", "cell_fill")) / (self.n_cells + 1)

@@ -61,7 +61,7 @@ class Arena:
     """One or more systems sharing a pair model and energy network, resident on one GPU."""
 
     def __init__(self, specs, shape: Shape, hypers, net: EnergyNet, device="cuda", clip=100.0,
-                 list_cap=None, r_nep=None, potential=None):
+                 list_cap=None, r_nep=None, potential=None, reverse_records=False):
         if not torch.cuda.is_available():
             raise _lib.NepxError("nepx needs a ROCm GPU (torch.cuda.is_available() is False)")
         self.device = torch.device(device)
@@ -77,6 +77,10 @@ class Arena:
         self.model = make_pair_model(shape, self.hypers, r_nep)
         self.cnet = net.cstruct() if net is not None else None
         self.clip = float(clip)
+        # reverse-order pair records: the fused pair kernel also writes each pair's j side at its
+        # reverse-row entry, so the gather reads it contiguously (a third less gather traffic), at the
+        # price of scattered writes in the pair kernel; measured slower per step, so off by default
+        self.reverse_records = bool(reverse_records)
         self._layout()
         self._alloc_state()
         self._alloc_list(list_cap)
@@ -163,6 +167,7 @@ class Arena:
     def _alloc_list(self, cap=None):
         dev, n = self.device, self.n
         cap = int(cap or self._estimate_cap())
+        builds = int(self.lt["flags"][_lib.FLAG_BUILDS].item()) if getattr(self, "lt", None) else 0
         i32 = lambda m: torch.zeros(m, dtype=torch.int32, device=dev)
         L = {
             "cell_count": i32(self.n_cells + 1), "cell_start": i32(self.n_cells + 1),
@@ -174,13 +179,19 @@ class Arena:
             "pair_i": i32(cap), "pair_j": i32(cap),
             "pair_t": torch.zeros(cap * 4, dtype=torch.float32, device=dev),
         }
+        if self.reverse_records:
+            # j side of each pair in reverse-row order, written by the fused pair kernel; zero tags
+            # (never a pass tag) until then
+            L.update({"rpos": i32(cap), "pair_e": i32(cap),
+                      "rrec": torch.zeros(cap * 8, dtype=torch.float32, device=dev)})
+        L["flags"][_lib.FLAG_BUILDS] = builds   # the build count carries over a regrown list
         self.lt = L
         self.pair_out = torch.zeros(cap * 12, dtype=torch.float32, device=dev)
         nl = _lib.NList()
         nl.cap = cap
         nl.n_cells_total = self.n_cells
         for k in _lib._NL_PTRS:
-            setattr(nl, k, L[k].data_ptr())
+            setattr(nl, k, L[k].data_ptr() if k in L else None)
         self.nl = nl
         self.cap = cap
 
@@ -301,8 +312,9 @@ class Arena:
             return
         self.prepare_pair()
         if not self.split:
-            _lib.check(_lib.lib.nepx_pair_fused(C.byref(self.model), C.byref(self.cnet), *args, self.clip,
-                                                _lib.ptr(self.pair_out), self._s(stream)), "nepx_pair_fused")
+            _lib.check(_lib.lib.nepx_pair_fused_list(C.byref(self.model), C.byref(self.cnet), C.byref(self.st),
+                                                     C.byref(self.nl), self.clip, _lib.ptr(self.pair_out),
+                                                     self._s(stream)), "nepx_pair_fused_list")
             return
         s = self._s(stream)
         _lib.check(_lib.lib.nepx_descriptor_fwd(C.byref(self.model), *args, _lib.ptr(self._G), s),

@@ -467,6 +467,9 @@ def main():
     ap.add_argument("--precision", default="auto", choices=["auto"] + sorted(NET_DESC),
                     help="network MFMA operands: " + "; ".join(f"{k}: {v}" for k, v in NET_DESC.items()))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--reverse-records", action="store_true",
+                    help="the pair kernel also writes each pair's j side in reverse-row order for the gather "
+                         "(less gather traffic, scattered writes in the pair kernel; off by default)")
     ap.add_argument("--long-steps", type=int, default=10000, help="nvt_long workload: steps of the run")
     ap.add_argument("--graph", type=int, default=1, help="capture the step in a hipGraph (1) or launch eagerly (0)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -523,7 +526,7 @@ def main():
         spec.skin = tr.best.skin
         tuning = tr.summary()
         print(f"[bench] autotune {tr.seconds:.1f} s: {tuning['best']}", file=sys.stderr, flush=True)
-    ar = nepx.Arena([spec], shape, hyp, net, device=dev)
+    ar = nepx.Arena([spec], shape, hyp, net, device=dev, reverse_records=a.reverse_records)
     runner = StepRunner(ar, use_graph=bool(a.graph))
     runner.init()
     overflows = run_recovering(runner, a.warmup)

@@ -126,10 +126,18 @@ typedef struct nepx_nlist {
                                         [2] P_L, [3] P, [4] builds, [5] longest row (when over 2048),
                                         [6] step frozen (set by the filter while [0] is set), [7] rebuild
                                         at the next build point, [8] 1 + a system with a row over 2048
-                                        (0: none), others zero */
+                                        (0: none), [9] filter pass count (the tag of this step's
+                                        reverse-order records), [10] tag of the last pass whose pair
+                                        kernel wrote them; others zero */
     int32_t* pair_i;                 /* cap: compacted pairs                       */
     int32_t* pair_j;
     float* pair_t;                   /* cap*4: minimum-image t = X_j - X_i (f32)  */
+    /* Optional (NULL: unused) reverse-order pair records, so that the gather reads the j side of
+       each pair contiguously along the reverse rows: */
+    int32_t* rpos;                   /* cap: reverse-row entry of list slot s (written by the build) */
+    int32_t* pair_e;                 /* cap: reverse-row entry of compacted pair c (by the filter)   */
+    float* rrec;                     /* cap*8, zero-initialised: at entry e, {F_i, tag}{tau_j, 0} of
+                                        its pair (by nepx_pair_fused_list)                          */
 } nepx_nlist;
 
 /* ---- library ---------------------------------------------------------- */
@@ -214,6 +222,14 @@ int nepx_pair_fused(const nepx_pair_model* m, const nepx_net* net, const float* 
                     const int32_t* pair_i, const int32_t* pair_j, const float* pair_t,
                     const int32_t* n_pairs_dev, int64_t max_pairs, float clip, float* out,
                     void* stream);
+/* nepx_pair_fused on the list's compacted pairs (pair_i, pair_j, pair_t, count flags[3]) and the
+ * state's orientations; when the list carries rpos / pair_e / rrec it also writes each pair's j side
+ * in reverse-row order (tagged with this filter pass), which nepx_forces_gather then reads
+ * contiguously.  Same per-pair results and the same gathered sums as nepx_pair_fused.  Replaces
+ * the descriptor + network + finite-difference part of ForceEvaluator.evaluate_interactions
+ * (ForceEvaluator.py:71-146) as nepx_pair_fused does. */
+int nepx_pair_fused_list(const nepx_pair_model* m, const nepx_net* net, const nepx_state* st, nepx_nlist* nl,
+                         float clip, float* out, void* stream);
 
 /* Descriptors of arbitrary site clouds, for training-set featurisation.
  * Replaces NEP.get_g_fast (05/NEP.py:171-219, 285-293) and the per-chunk loop of

@@ -207,3 +207,32 @@ def test_non_finite_state_stops_only_its_system(gpu):
     a, b = alone.system_arrays(0), ar.system_arrays(1)
     for k in ("x", "v", "q", "p", "F", "T"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_reverse_record_gather_equals_slot_map_gather(gpu):
+    """The gather's two ways to the j side of each pair — the fused kernel's reverse-order records,
+    read along each particle's reverse row, and the per-pair records reached through the slot ->
+    compacted-index map — sum the same values in the same order: forces, torques and energies agree
+    bit for bit (tet10 and cube8, list with inactive slots)."""
+    import nepx
+    from nepx import _lib
+    from nepx.configs import make_system
+    for shape_name, n, phi in (("tet10", 1500, 0.45), ("cube8", 512, 0.3)):
+        spec = make_system(shape_name, n, phi, seed=6)
+        net = nepx.EnergyNet.random(40, [64, 64], seed=106, device=gpu)
+        ar = nepx.Arena([spec], nepx.get_shape(shape_name), (11, 6, 4), net, device=gpu, reverse_records=True)
+        ar.init()
+        ar.filter_pairs()
+        ar.pair_fused()
+        ar.gather()
+        torch.cuda.synchronize()
+        f = ar.flags()
+        assert f[_lib.FLAG_TAG] > 0 and f[_lib.FLAG_RTAG] == f[_lib.FLAG_TAG]
+        assert f[_lib.FLAG_PL] > ar.n_pairs()          # some list slots are outside the cutoff this step
+        a = ar.system_arrays(0)
+        ar.lt["flags"][_lib.FLAG_RTAG:_lib.FLAG_RTAG + 1].fill_(-1)   # no reverse records this pass
+        ar.gather()
+        torch.cuda.synchronize()
+        b = ar.system_arrays(0)
+        for k in ("F", "T", "upot"):
+            assert np.array_equal(a[k], b[k]), (shape_name, k)
