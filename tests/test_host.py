@@ -573,3 +573,34 @@ def test_bf16_forms_refused_for_relu_networks():
     with pytest.raises(ValueError):
         nepx.EnergyNet.random(40, [64, 64], seed=1, act="relu", device=None, precision="bf16x3")
     assert nepx.EnergyNet.random(40, [64, 64], seed=1, act="tanh", device=None, precision="bf16").precision == "bf16"
+
+
+def test_poisson_trace_builds_job_arrays_lazily():
+    """A long job trace holds only seeds until a job is looked at; its arrays, built on demand,
+    are the same after a release and rebuild, and a deep copy does not build them."""
+    import copy
+    from nepx.serve import LazySpec, poisson_jobs
+    jobs = poisson_jobs(rate=20.0, duration=30.0, seed=2025, shapes=("cube8", "tet10", "cyl12"))
+    assert len(jobs) > 400 and all(isinstance(j.spec, LazySpec) and j.spec._spec is None for j in jobs)
+    j = jobs[7]
+    n = j.n                                  # the size is known without building the arrays
+    assert j.spec._spec is None and n > 0
+    x0 = j.spec.x.copy()
+    assert j.spec._spec is not None and x0.shape == (n, 3)
+    c = copy.deepcopy(jobs[8])
+    assert c.spec._spec is None
+    j.spec.release()
+    assert j.spec._spec is None and np.array_equal(j.spec.x, x0)
+
+
+def test_isa_phase_counter_finds_both_site_loops():
+    """The static per-phase instruction counter of the four-lane fused kernel (gfx950 assembly)."""
+    import subprocess
+    import sys
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "anisotropic-ml_amd", "isa_phases.py"), "--sites", "8"],
+                         capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    rows = {l.split("|")[1].strip(): l for l in out.stdout.splitlines() if l.startswith("| ")}
+    assert "forward (per trip)" in rows and "adjoint (per trip)" in rows and "network" in rows
+    total = int(out.stdout.strip().splitlines()[-1].split("|")[-2])
+    assert 2000 < total < 6000, total
