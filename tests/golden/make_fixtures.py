@@ -5,9 +5,11 @@ Input: 64 pair site clouds sampled (seed 0) from the reference's data file
 with numpy's safe loader).  Expected output: the FP64 oracle descriptor with
 the cube_v2 model hypers (10, 8, 4) and r_nep = 4.5
 (nep_simulator/DescriptorGenerator.py:63-79, runner.sh:2).
-Also copies the reference's initial configuration
+Also copies, unchanged, the reference's initial configuration
 nep_simulator/init/th_init_N200.gsd (200 tetrahedra, box 33, written by
-HOOMD-blue 4.7.0) unchanged as tests/golden/th_init_N200.gsd.
+HOOMD-blue 4.7.0) as tests/golden/th_init_N200.gsd, and the label potential's
+bead table 01/cube_structure.gsd (152 beads of one cube, read by
+01/SamplerUtils.py) as tests/golden/cube_structure.gsd.
 Run in a container that holds the reference checkout:
     python tests/golden/make_fixtures.py
 """
@@ -22,6 +24,7 @@ from oracle import nep_oracle as O  # noqa: E402
 
 SRC = "/root/reference/01/datadir/x_12pts_500_s42.npy"
 INIT = "/root/reference/nep_simulator/init/th_init_N200.gsd"
+BEADS = "/root/reference/01/cube_structure.gsd"
 
 if __name__ == "__main__":
     x = np.load(SRC, allow_pickle=False)
@@ -32,3 +35,4 @@ if __name__ == "__main__":
     print("wrote", clouds.shape, G.shape)
     import shutil
     shutil.copyfile(INIT, os.path.join(HERE, "th_init_N200.gsd"))
+    shutil.copyfile(BEADS, os.path.join(HERE, "cube_structure.gsd"))
