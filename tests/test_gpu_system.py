@@ -236,3 +236,28 @@ def test_reverse_record_gather_equals_slot_map_gather(gpu):
         b = ar.system_arrays(0)
         for k in ("F", "T", "upot"):
             assert np.array_equal(a[k], b[k]), (shape_name, k)
+
+
+def test_md_step_c_entry_equals_arena_steps(gpu):
+    """One C call per step (nepx_md_step: list build, filter, fused pair kernel, gather, integrator)
+    gives the same trajectory bit for bit as the Arena's component calls."""
+    import ctypes as C
+    from nepx import _lib
+    a, _, _ = _arena(gpu, "tet10", 700, 0.35, seed=9)
+    b, _, _ = _arena(gpu, "tet10", 700, 0.35, seed=9)
+    a.init()
+    b.init()
+    b.prepare_pair()
+    stream = torch.cuda.current_stream()
+    for _ in range(10):
+        a.step()
+        _lib.check(_lib.lib.nepx_md_step(C.byref(b.st), C.byref(b.nl), C.byref(b.model), C.byref(b.cnet),
+                                         C.c_float(b.clip), _lib.ptr(b.pair_out), _lib.stream_handle(stream)),
+                   "nepx_md_step")
+    torch.cuda.synchronize()
+    a.check()
+    b.check()
+    x, y = a.system_arrays(0), b.system_arrays(0)
+    for k in ("x", "v", "q", "p", "F", "T"):
+        assert np.array_equal(x[k], y[k]), k
+    assert a.device_steps() == b.device_steps() == 10
