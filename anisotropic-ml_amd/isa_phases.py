@@ -26,12 +26,12 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def device_asm(out):
+def device_asm(out, source=None):
     sys.path.insert(0, HERE)
     import build
     flags = [f for f in build.FLAGS if f != "-fPIC"]
     cmd = ([build.HIPCC] + flags + build.SOURCE_FLAGS.get("nepx_quad.hip", []) +
-           ["--cuda-device-only", "-S", os.path.join(build.CSRC, "nepx_quad.hip"), "-o", out])
+           ["--cuda-device-only", "-S", source or os.path.join(build.CSRC, "nepx_quad.hip"), "-o", out])
     subprocess.run(cmd, check=True, capture_output=True)
 
 
@@ -81,10 +81,11 @@ def main():
     ap.add_argument("--hypers", type=int, nargs=3, default=(11, 6, 4))
     ap.add_argument("--prec", type=int, default=1, help="0 f32, 1 f16x3, 2 bf16, 3 bf16x3")
     ap.add_argument("--act", type=int, default=0, help="0 tanh, 1 relu")
+    ap.add_argument("--source", default=None, help="another copy of nepx_quad.hip to count (default: the build's)")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as d:
         out = os.path.join(d, "quad.s")
-        device_asm(out)
+        device_asm(out, a.source)
         lines = open(out).read().split("\n")
     body = kernel_lines(lines, a.sites, a.hypers, a.prec, a.act)
     vgpr = next((l.split()[-1] for l in body if ".amdhsa_next_free_vgpr" in l), "?")
