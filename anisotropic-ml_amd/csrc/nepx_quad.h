@@ -87,7 +87,6 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     constexpr int PW = 16;                          // pairs per wave
     static_assert(LD <= Q::TS, "descriptor rows fit in the wave's exchange area");
     __shared__ __attribute__((aligned(16))) float xch[Q::BLOCK / 64][PW * Q::TS];
-    __shared__ float etile[Q::BLOCK / 4];
     __shared__ float4 stab[Cfg::ND];
     using R = NetRun<N, 1, PREC>;
     __shared__ __attribute__((aligned(16))) float wl[weight_lds_floats<N, PREC>()];
@@ -223,7 +222,8 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
 #pragma unroll
             for (int l = 0; l < LMAX; ++l) ga[n][l] = quad_sum(ga[n][l]);
         wave_lds_order();  // the exchange reads are done before the rows overwrite the area
-        if (q == 0) {  // rows of pairs past the end hold finite values of the stand-in pair; unused
+        {  // every lane of the quad holds the same sums and writes them (no branch); rows of pairs
+           // past the end hold finite values of the stand-in pair and are unused
 #pragma unroll
             for (int n = 0; n <= Cfg::NRAD; ++n) grow[n] = gr[n];
 #pragma unroll
@@ -233,6 +233,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
         }
         wave_lds_order();
         // ---- energy network on the wave's 16 pairs: rows in, dE/dG rows out ----
+        float epair;
         {
             float* row = xw + (lane & 15) * LD;
             v4f gin[1][R::IC];
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             v4f dg[1][N::MD];
             float y[1];
             R::run(W, WH, ACT, gin, scale, dg, y);
-            if (lane < 16) etile[wv * 16 + lane] = en_min + scale * y[0];
+            epair = __shfl(en_min + scale * y[0], slot);  // pair p's energy is in lanes with (lane & 15) == p
             scatter_chunks<N::MD>(row, true, Cfg::D, dg[0]);
         }
         wave_lds_order();
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
         float dGr[Cfg::NRAD + 1];
 #pragma unroll
         for (int n = 0; n <= Cfg::NRAD; ++n) dGr[n] = grow[n];
-        const float E = etile[wv * 16 + slot];
+        const float E = epair;
         // B = dE/dA = 2 K^2 dE/dg_ang[n][l] A, in place (rows n >= NA stay zero)
 #pragma unroll
         for (int d = 0; d < LC; ++d) {

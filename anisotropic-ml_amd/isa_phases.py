@@ -8,8 +8,9 @@ kernel's main loop (one iteration = 16 pairs per wave):
   setup     blocks from the loop head to the forward site loop (pair loads, rotation matrices,
             pair geometry)
   forward   the forward site loop (one site per lane per trip; trips = sites per lane)
-  network   the blocks between the two site loops that issue the network's 16x16x32 MFMAs
-  between   the other blocks between the site loops (descriptor epilogue, adjoint prelude)
+  network   the blocks between the two site loops that issue the network's 16x16x32 MFMAs (with
+            whatever the compiler schedules into the same basic block, e.g. the descriptor epilogue)
+  between   the other blocks between the site loops (adjoint prelude, descriptor epilogue if apart)
   adjoint   the adjoint site loop (same trip count)
   tail      the rest of the iteration (per-pair sums, clip, stores)
 
