@@ -83,7 +83,9 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
     static_assert(Q::OK, "shape not supported by the four-lane kernel");
     constexpr int NC = Q::NC, LC = Q::LC, NA = Cfg::NANG + 1, NLM = Cfg::NLM, LMAX = Cfg::LMAX;
     constexpr int GA = Cfg::NRAD + 1;              // first angular entry of a descriptor row
-    constexpr int LD = (Cfg::D + 3) / 4 * 4 + 4;   // descriptor row stride (16-byte aligned)
+    // descriptor row stride: whole 16-entry chunks of the network's dE/dG tiles (written without
+    // a per-lane bound) plus 4 floats
+    constexpr int LD = N::MD * 16 + 4;
     constexpr int PW = 16;                          // pairs per wave
     static_assert(LD <= Q::TS, "descriptor rows fit in the wave's exchange area");
     __shared__ __attribute__((aligned(16))) float xch[Q::BLOCK / 64][PW * Q::TS];
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             float y[1];
             R::run(W, WH, ACT, gin, scale, dg, y);
             epair = __shfl(en_min + scale * y[0], slot);  // pair p's energy is in lanes with (lane & 15) == p
-            scatter_chunks<N::MD>(row, true, Cfg::D, dg[0]);
+            scatter_chunks<N::MD>(row, true, 16 * N::MD, dg[0]);  // padding units' dE/dG are zero
         }
         wave_lds_order();
         // ---- adjoint ----
