@@ -129,7 +129,8 @@ def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
                              0.5 * float(np.sum(spec.mass[:, None] * spec.v ** 2)),
                              float(np.sum(O.rot_ke(spec.q, spec.p, spec.moi))), spec.ndof, spec.rndof, spec.dt,
                              spec.tau, spec.kT)
-    probe = (spec.x[:2], spec.q[:2], np.array([[0, 1]], np.int64), sh, spec.L, hyp, layers, "tanh", 0.0, 1.0,
+    act = getattr(net, "act", "tanh")
+    probe = (spec.x[:2], spec.q[:2], np.array([[0, 1]], np.int64), sh, spec.L, hyp, layers, act, 0.0, 1.0,
              1e-5, 1e-5)
     pool = _start_pool(workers, probe) if workers > 1 else None
     if pool is None:
@@ -138,7 +139,7 @@ def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
     try:
         if spec.n <= 4096:
             run = lambda st, th, n: R.reference_run(st, th, sh, spec.L, hyp, layers, spec.dt, spec.kT, spec.tau,
-                                                    spec.moi, spec.ndof, spec.rndof, n, pool=pool,
+                                                    spec.moi, spec.ndof, spec.rndof, n, act=act, pool=pool,
                                                     chunk=max(64, 5000 // workers))
             t0 = time.perf_counter()
             state, thermo, _ = run(state, thermo, 1)                     # one step sizes the sample
@@ -164,7 +165,7 @@ def cpu_baseline(spec, shape_name, hyp, net, target_s=15.0):
         t0 = time.perf_counter()
         while time.perf_counter() - t0 < target_s and done < len(sample):
             batch = sample[done:done + chunk * workers]
-            R.reference_forces(spec.x, spec.q, sh, spec.L, hyp, layers, pairs=batch, pool=pool, chunk=chunk)
+            R.reference_forces(spec.x, spec.q, sh, spec.L, hyp, layers, act, pairs=batch, pool=pool, chunk=chunk)
             done += len(batch)
         t_pairs = time.perf_counter() - t0
     finally:
@@ -243,6 +244,55 @@ def serve_bench(a):
     if a.serve_plain_window:
         res["plain_window"] = run_plain(plain_jobs, classes, time_limit=a.serve_seconds, chunk=a.serve_chunk,
                                         device=dev, log=log)
+    return res
+
+
+def reference_run_bench(a):
+    """The reference's documented run (sim_input.py:10: --hypers 10 10 10 4.5 --shape th_v1, a
+    TheEnergyNet(150, 3, 121) ReLU checkpoint) through the drop-in surface: NVT() with the setter
+    sequence of sim_input.py:50-59 on the reference's init/th_init_N200.gsd (200 tetrahedra, box 33,
+    copied to tests/golden), kT 0.5, dt 0.005, a 121->150x4->1 ReLU network with seeded weights (the
+    checkpoint itself is not shipped) on the pair kernels without a fused instance (descriptor ->
+    network -> adjoint through HBM).  Sim.run(steps) is timed end to end."""
+    import contextlib
+    import torch
+    import nepx
+    from nepx.NVT import NVT
+    dev = torch.device("cuda:0")
+    net = nepx.EnergyNet.random(121, [150, 150, 150, 150], seed=21, act="relu", en_min=-6.7, en_max=15.0,
+                                device=dev)
+    sim = NVT()
+    with contextlib.redirect_stdout(sys.stderr):   # Sim prints its timers; stdout carries the JSON line
+        sim.setDevice(0)
+        sim.initializeSystem(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                          "th_init_N200.gsd"))
+        sim.setNepDescriptors([10, 10, 10, 4.5])
+        sim.setEvaluator(net=net)
+        sim.setShape("th_v1")
+        sim.setNeighborList()
+        sim.setkT(0.5)
+        sim.setdt(0.005)
+        sim.run(a.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run(a.steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    ar = sim.arena
+    n = ar.specs[0].n
+    res = {
+        "metric": "particle-steps/s, the reference's documented run (200 th_v1 tetrahedra, D 121)",
+        "value": n * a.steps / wall, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True,
+        "dtype": f"f32 pair math ({NET_DESC[net.precision]}), f64 integrator",
+        "data": "the reference's init/th_init_N200.gsd; network 121->150x4->1 ReLU, seeded weights",
+        "config": {"workload": "200 th_v1 tetrahedra (th_init_N200.gsd, box 33) hypers=(10, 10, 10) r_nep 4.5 "
+                               "rc 5.75 FFNN 121->150->150->150->150->1 relu, NVT kT 0.5 dt 0.005",
+                   "pairs_per_step": ar.n_pairs(), "fused_kernel": bool(not ar.split),
+                   "steps_per_graph": sim.runner.K if sim.runner is not None else 1},
+    }
+    if not a.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(ar.specs[0], "th_v1", (10, 10, 10), net)
     return res
 
 
@@ -496,6 +546,9 @@ def main():
         return
     if a.workload == "ffnn":
         print(json.dumps(ffnn_bench(a)))
+        return
+    if a.workload == "ref_th200":
+        print(json.dumps(reference_run_bench(a)))
         return
     if a.workload == "replicas256":
         print(json.dumps(replicas_bench(a)))
