@@ -390,15 +390,15 @@ __device__ __forceinline__ void desc_finish(const float (&A)[Cfg::NANG + 1][LR::
 // range LR, c_lm = sum_n B_nlm f_n(rho_a) and d_lm = sum_n B_nlm f_n'(rho_a).
 // Adds sum_lm d_lm Z_lm to dEdrho and returns
 //   (gx, gy, gz) = grad sum_lm c_lm Z_lm  (Z_lm extended to homogeneous polynomials of degree l)
-//   proj         = sum_lm l c_lm Z_lm(u_a),
 // using  d/dx Q_l^m = x Q_{l-1}^{m+1},  d/dy Q_l^m = y Q_{l-1}^{m+1},  d/dz Q_l^m = (l+m) Q_{l-1}^m,
 //        d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}),
-// with the sums over l taken first for each m, so each (l, m) costs a few FMAs.
+// with the sums over l taken first for each m, so each (l, m) costs a few FMAs.  The radial
+// projection sum_lm l c_lm Z_lm(u_a) is not summed here: by Euler's theorem on the degree-l
+// terms it equals u_a . grad at |u_a| = 1 (site_dEdr).
 template <class LR, class CD>
 __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, CD&& cd,
-                                                  float& dEdrho, float& gx, float& gy, float& gz, float& proj) {
+                                                  float& dEdrho, float& gx, float& gy, float& gz) {
     float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f;
-    proj = 0.f;
 #pragma unroll
     for (int mm = 0; mm <= LR::L1; ++mm) {
         float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
@@ -407,7 +407,6 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
             const float ql = h.q(l, mm);
             const float qa = (mm + 1 <= l - 1) ? h.q(l - 1, mm + 1) : 0.f;
             const float qc = (mm <= l - 1) ? (float)(l + mm) * h.q(l - 1, mm) : 0.f;
-            float pl = 0.f;
 #pragma unroll
             for (int part = 0; part < (mm ? 2 : 1); ++part) {
                 const int o = lm_slot(l, mm, part) - LR::OFF;
@@ -415,12 +414,10 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
                 float c, d;
                 cd(o, c, d);
                 dEdrho += d * z;
-                pl += c * z;
                 if (mm + 1 <= l - 1) U[part] += qa * c;
                 if (mm <= l - 1) V[part] += qc * c;
                 if (mm >= 1) W[part] += ql * c;
             }
-            proj += (float)l * pl;
         }
         if (mm == 0) {
             E1 += U[0];
@@ -438,9 +435,11 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
     gz = Gz;
 }
 
-// dE/dr_a = dE/drho_a u_a / r_nep + (grad - proj u_a) / |r_a|.
+// dE/dr_a = dE/drho_a u_a / r_nep + (grad - (u_a . grad) u_a) / |r_a|, where u_a . grad =
+// sum_lm l c_lm Z_lm(u_a) (Euler's theorem on the degree-l terms of the homogeneous extension).
 __device__ __forceinline__ void site_dEdr(const SiteGeo& s, float dEdrho, float inv_rn, float gx, float gy, float gz,
-                                          float proj, float& ox, float& oy, float& oz) {
+                                          float& ox, float& oy, float& oz) {
+    const float proj = s.ux * gx + s.uy * gy + s.uz * gz;
     const float rad = dEdrho * inv_rn;
     ox = rad * s.ux + s.ir * (gx - proj * s.ux);
     oy = rad * s.uy + s.ir * (gy - proj * s.uy);
@@ -509,7 +508,7 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
             });
             Harm<LR::L1> h;
             h.eval(s.ux, s.uy, s.uz);
-            float gx, gy, gz, proj;
+            float gx, gy, gz;
             angular_site_grad<LR>(h, s.ux, s.uy,
                                   [&](int o, float& c, float& d) {
                                       c = 0.f;
@@ -520,9 +519,9 @@ __device__ __forceinline__ void desc_adjoint(const nepx_pair_model& m, const Pai
                                           d += A[n][o] * da[n];
                                       }
                                   },
-                                  dEdrho, gx, gy, gz, proj);
+                                  dEdrho, gx, gy, gz);
             float ga_x, ga_y, ga_z;
-            site_dEdr(s, dEdrho, inv_rn, gx, gy, gz, proj, ga_x, ga_y, ga_z);
+            site_dEdr(s, dEdrho, inv_rn, gx, gy, gz, ga_x, ga_y, ga_z);
             gx_s += ga_x;
             gy_s += ga_y;
             gz_s += ga_z;

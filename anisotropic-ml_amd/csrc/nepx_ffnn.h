@@ -322,21 +322,29 @@ __device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, cons
 #pragma unroll
             for (int t = 0; t < NT; ++t) acc[t][mt] = v4f{b4.x, b4.y, b4.z, b4.w};
         }
+        // every k-step's operand split first, then the layer's products as one block of matrix
+        // instructions (the split VALU work of one layer does not interleave with its products;
+        // 8 % faster for the 128-wide cylinder network, neutral for 64-wide ones)
+        constexpr int KSMAX = (N::DPH > N::HP ? N::DPH : N::HP) / 32;
+        V bh[KSMAX][NT], bl[KSMAX][NT];
 #pragma unroll
-        for (int s = 0; s < ks; ++s) {
-            V bh[NT], bl[NT];
+        for (int s = 0; s < ks; ++s)
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                if (k == 0) split8<PREC>(gin[t][2 * s], gin[t][2 * s + 1], 1.f, bh[t], bl[t]);
-                else split8<PREC>(h[k - 1][t][2 * s], h[k - 1][t][2 * s + 1], 1.f, bh[t], bl[t]);
+                if (k == 0) split8<PREC>(gin[t][2 * s], gin[t][2 * s + 1], 1.f, bh[s][t], bl[s][t]);
+                else split8<PREC>(h[k - 1][t][2 * s], h[k - 1][t][2 * s + 1], 1.f, bh[s][t], bl[s][t]);
             }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int s = 0; s < ks; ++s) {
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) {
                 const int idx = base + (mt * ks + s) * 64 + lane;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t][mt] = mma_split<PREC>(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
+                for (int t = 0; t < NT; ++t) acc[t][mt] = mma_split<PREC>(A8, idx, lo_off, bh[s][t], bl[s][t], acc[t][mt]);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -378,19 +386,23 @@ __device__ __forceinline__ void ffnn_tiles_h(const float* __restrict__ W32, cons
         for (int t = 0; t < NT; ++t)
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) acc[t][mt] = v4f{0.f, 0.f, 0.f, 0.f};
+        V bh[ks][NT], bl[ks][NT];
+#pragma unroll
+        for (int s = 0; s < ks; ++s)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) split8<PREC>(delta[t][2 * s], delta[t][2 * s + 1], kUp, bh[s][t], bl[s][t]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int s = 0; s < ks; ++s) {
-            V bh[NT], bl[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) split8<PREC>(delta[t][2 * s], delta[t][2 * s + 1], kUp, bh[t], bl[t]);
 #pragma unroll
             for (int mt = 0; mt < N::MT; ++mt) {
                 if (mt >= mtn) break;
                 const int idx = base + (mt * ks + s) * 64 + lane;
 #pragma unroll
-                for (int t = 0; t < NT; ++t) acc[t][mt] = mma_split<PREC>(A8, idx, lo_off, bh[t], bl[t], acc[t][mt]);
+                for (int t = 0; t < NT; ++t) acc[t][mt] = mma_split<PREC>(A8, idx, lo_off, bh[s][t], bl[s][t], acc[t][mt]);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);
         if (k > 0) {
 #pragma unroll
             for (int t = 0; t < NT; ++t)

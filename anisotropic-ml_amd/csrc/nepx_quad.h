@@ -192,6 +192,8 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             for (int c = 0; c < NC; ++c) xf[c] = *reinterpret_cast<const v4f*>(xp + 16 * c + 4 * q);
 #pragma unroll
             for (int d = 0; d < LC; ++d) xz[d] = *reinterpret_cast<const v4f*>(xp + 16 * (NC + d) + 4 * q);
+            // the accumulation issues as one block after the exchange (as in the adjoint below)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -199,6 +201,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
 #pragma unroll
                     for (int d = 0; d < LC; ++d)
                         Af[c][d] = __builtin_amdgcn_mfma_f32_4x4x1f32(xf[c][k], xz[d][k], Af[c][d], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
         // ---- descriptor row: g_rad, g_ang[n][l] = sum_m K^2 A^2 ----
 #pragma unroll
@@ -279,7 +282,11 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
                     da[n] = df;
                 }
             });
+            // the c/d products issue as one block, with no VALU work scheduled between them, while
+            // the other wave on the SIMD issues its VALU work (4 % faster at tet10 and cyl12 than
+            // letting the compiler interleave them)
             v4f cc[LC], dd[LC];
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int d = 0; d < LC; ++d) {
                 cc[d] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -293,18 +300,19 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
                     cc[d] = __builtin_amdgcn_mfma_f32_4x4x1f32(b, fa[n], cc[d], 0, 0, 0);
                     dd[d] = __builtin_amdgcn_mfma_f32_4x4x1f32(b, da[n], dd[d], 0, 0, 0);
                 }
+            __builtin_amdgcn_sched_barrier(0);
             Harm<LMAX> h;
             h.eval(sg.ux, sg.uy, sg.uz);
-            float gx, gy, gz, proj;
+            float gx, gy, gz;
             angular_site_grad<LRange<1, LMAX>>(
                 h, sg.ux, sg.uy,
                 [&](int o, float& c, float& dv) {
                     c = cc[o / 4][o % 4];
                     dv = dd[o / 4][o % 4];
                 },
-                dEdrho, gx, gy, gz, proj);
+                dEdrho, gx, gy, gz);
             float ex, ey, ez;
-            site_dEdr(sg, dEdrho, inv_rn, gx, gy, gz, proj, ex, ey, ez);
+            site_dEdr(sg, dEdrho, inv_rn, gx, gy, gz, ex, ey, ez);
             const float tx = sg.ay * ez - sg.az * ey, ty = sg.az * ex - sg.ax * ez, tz = sg.ax * ey - sg.ay * ex;
             const float wj = bj ? 1.f : 0.f, wi = 1.f - wj;
             ig_x += wi * ex; ig_y += wi * ey; ig_z += wi * ez;
