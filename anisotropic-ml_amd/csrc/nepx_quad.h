@@ -192,8 +192,10 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             for (int c = 0; c < NC; ++c) xf[c] = *reinterpret_cast<const v4f*>(xp + 16 * c + 4 * q);
 #pragma unroll
             for (int d = 0; d < LC; ++d) xz[d] = *reinterpret_cast<const v4f*>(xp + 16 * (NC + d) + 4 * q);
-            // the accumulation issues as one block after the exchange (as in the adjoint below)
+            // the accumulation issues as one block after the exchange (as in the adjoint below), at
+            // the lower wave priority
             __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -202,6 +204,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
                     for (int d = 0; d < LC; ++d)
                         Af[c][d] = __builtin_amdgcn_mfma_f32_4x4x1f32(xf[c][k], xz[d][k], Af[c][d], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_setprio(1);
         }
         // ---- descriptor row: g_rad, g_ang[n][l] = sum_m K^2 A^2 ----
 #pragma unroll
@@ -284,9 +287,12 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             });
             // the c/d products issue as one block, with no VALU work scheduled between them, while
             // the other wave on the SIMD issues its VALU work (4 % faster at tet10 and cyl12 than
-            // letting the compiler interleave them)
+            // letting the compiler interleave them).  The wave drops to the lower priority for the
+            // block and raises it after, so that the VALU work, the bound, issues first when both
+            // waves of a SIMD are ready (4 % faster at tet10)
             v4f cc[LC], dd[LC];
             __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int d = 0; d < LC; ++d) {
                 cc[d] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -301,6 +307,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
                     dd[d] = __builtin_amdgcn_mfma_f32_4x4x1f32(b, da[n], dd[d], 0, 0, 0);
                 }
             __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_setprio(1);
             Harm<LMAX> h;
             h.eval(sg.ux, sg.uy, sg.uz);
             float gx, gy, gz;
