@@ -119,6 +119,106 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, c
     }
 }
 
+// FP32 network for few pairs: one 16-pair tile per block, and the MT waves of the block take one
+// 16-unit block of every layer each (wave w computes units 16w..16w+15 of every hidden layer and
+// of every backward delta, over all k-steps), so that a tile's chain of layers is spread over MT
+// waves instead of running on one.  The layer outputs go through LDS to the next layer's B
+// operands.  Every accumulator takes the same products in the same order as ffnn_tiles, and the
+// output sum is formed the same way, so E and dE/dG equal k_ffnn's bit for bit.
+template <class N, int ACT>
+__global__ __launch_bounds__(N::MT * 64) void k_ffnn_coop(const float* __restrict__ W, float en_min, float scale,
+                                                         int D, const float* __restrict__ G,
+                                                         const int32_t* __restrict__ np_dev, int64_t maxp,
+                                                         float* __restrict__ E, float* __restrict__ dEdG) {
+    constexpr NetLayout Lay = N::layout();
+    constexpr int MT = N::MT;
+    __shared__ v4f xs[2][MT][64];  // one layer's outputs: block c of lane l at [c][l]
+    const int P = n_pairs(np_dev, maxp);
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));  // keeps the weight reads inside the tile loop
+    const int g = lane >> 4;
+    const int mt = threadIdx.x >> 6;  // this wave's 16-unit block
+    const int nt = (P + 15) / 16;
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int p = tile * 16 + (lane & 15);
+        v4f gin[N::MD];
+        gather_chunks<N::MD>(G + (size_t)p * D, p < P, D, gin);
+        v4f hown[N::NL];  // this wave's block of every hidden layer
+        int buf = 0;
+#pragma unroll
+        for (int k = 0; k < N::NL; ++k) {
+            const float* Wf = W + Lay.fwd_off(k);
+            const int ks = (k == 0) ? N::KD : N::KH;
+            const float4 b4 = *reinterpret_cast<const float4*>(W + Lay.bias_off(k) + 16 * mt + 4 * g);
+            v4f acc = v4f{b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int c = 0; c < ks / 4; ++c) {
+                const v4f x = (k == 0) ? gin[c < N::MD ? c : 0] : xs[buf ^ 1][c][lane];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Wf[(mt * ks + 4 * c + r) * 64 + lane], x[r], acc, 0,
+                                                              0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hown[k][r] = act_fwd(acc[r], ACT);
+            xs[buf][mt][lane] = hown[k];
+            __syncthreads();
+            buf ^= 1;
+        }
+        // output layer: every wave forms y from the whole last layer, in ffnn_tiles' order
+        const float* wo = W + Lay.wout_off();
+        float y = 0.f;
+#pragma unroll
+        for (int c = 0; c < MT; ++c) {
+            const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * c + 4 * g);
+            const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+            const v4f h = xs[buf ^ 1][c][lane];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) y += wv[r] * h[r];
+        }
+        y += __shfl_xor(y, 16);
+        y += __shfl_xor(y, 32);
+        y += W[Lay.bout_off()];
+        if (mt == 0 && p < P && lane < 16) E[p] = en_min + scale * y;
+        v4f delta;
+        {
+            const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * mt + 4 * g);
+            const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) delta[r] = scale * wv[r] * act_der(hown[N::NL - 1][r], ACT);
+        }
+        // backward: layer k's deltas through LDS, then this wave's block of layer k-1 (k = 0: dE/dG)
+#pragma unroll
+        for (int k = N::NL - 1; k >= 0; --k) {
+            xs[buf][mt][lane] = delta;
+            __syncthreads();
+            buf ^= 1;
+            if (k == 0 && mt >= N::MD) break;
+            const float* Wb = W + Lay.bwd_off(k);
+            v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < N::KH / 4; ++c) {
+                const v4f x = xs[buf ^ 1][c][lane];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Wb[(mt * N::KH + 4 * c + r) * 64 + lane], x[r], acc,
+                                                              0, 0, 0);
+            }
+            if (k > 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) delta[r] = acc[r] * act_der(hown[k - 1][r], ACT);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = 16 * mt + 4 * g + r;
+                    if (p < P && e < D) dEdG[(size_t)p * D + e] = acc[r];
+                }
+            }
+        }
+        __syncthreads();  // the last reads of xs are done before the next tile writes it
+    }
+}
+
 // Fused: descriptor -> LDS -> MFMA network -> LDS -> adjoint.  Each block
 // takes 256 consecutive pairs per iteration (64 per wave), so every wave of a
 // block runs the same number of iterations and the barriers are uniform.
@@ -555,6 +655,24 @@ static int launch_ffnn(const nepx_net* net, int grid, float scale, int d, const 
     return NEPX_OK;
 }
 
+// Few tiles (up to four per CU): the FP32 network runs one tile per block on k_ffnn_coop, which gives
+// the same results as k_ffnn.
+template <class T>
+static bool launch_ffnn_coop(const nepx_net* net, float scale, int d, const float* G, const int32_t* np_dev,
+                             int64_t maxp, float* E, float* dEdG, void* stream) {
+    const bool f32 = net->precision == NEPX_NET_F32 || T::HP % 32 != 0;
+    const int64_t tiles = (maxp + 15) / 16;
+    if (!f32 || tiles > 4 * (int64_t)device_cus()) return false;
+    const dim3 grid((unsigned)tiles), block(T::MT * 64);
+    if (net->act == NEPX_ACT_TANH)
+        hipLaunchKernelGGL((k_ffnn_coop<T, NEPX_ACT_TANH>), grid, block, 0, as_stream(stream), net->w_packed,
+                           net->en_min, scale, d, G, np_dev, maxp, E, dEdG);
+    else
+        hipLaunchKernelGGL((k_ffnn_coop<T, NEPX_ACT_RELU>), grid, block, 0, as_stream(stream), net->w_packed,
+                           net->en_min, scale, d, G, np_dev, maxp, E, dEdG);
+    return true;
+}
+
 extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int32_t* np_dev, int64_t maxp,
                                  float* E, float* dEdG, void* stream) {
     NEPX_REQUIRE(net && net->w_packed, NEPX_EINVAL, "null network");
@@ -567,6 +685,10 @@ extern "C" int nepx_ffnn_fwd_bwd(const nepx_net* net, const float* G, const int3
     const float scale = net->en_max - net->en_min;
 #define X(d, w, h, T, NT)                                                                        \
     if (net->d_in == d && net->width == w && net->n_hidden == h) {                               \
+        if (launch_ffnn_coop<T>(net, scale, d, G, np_dev, maxp, E, dEdG, stream)) {              \
+            NEPX_LAUNCH_CHECK();                                                                 \
+            return NEPX_OK;                                                                      \
+        }                                                                                        \
         const int grid = grid_for_cap((maxp + 16 * NT - 1) / (16 * NT), kBlock / 64,            \
                                       (int64_t)device_cus() * tuning().net_blocks_per_cu);       \
         return launch_ffnn<T, NT>(net, grid, scale, d, G, np_dev, maxp, E, dEdG, stream);       \

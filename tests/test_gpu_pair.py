@@ -95,6 +95,38 @@ def test_ffnn_matches_oracle(gpu, d_in, widths, act, precision):
     assert eG < tG
 
 
+@pytest.mark.parametrize("d_in,widths,act", [(121, [150] * 4, "relu"), (47, [150] * 4, "relu"),
+                                             (40, [64, 64], "tanh"), (60, [128, 128], "tanh")])
+def test_ffnn_few_pairs_kernel_equals_wide_kernel(gpu, d_in, widths, act):
+    """Up to four 16-pair tiles per CU the FP32 network runs one tile per block, the layers' unit
+    blocks spread over the block's waves; it gives the one-wave-per-tile kernel's results bit for
+    bit.  The capacity argument picks the kernel (241 pairs either way)."""
+    import ctypes as C
+    import nepx
+    from nepx import _lib
+    rng = np.random.default_rng(12)
+    P = 241
+    G = np.concatenate([rng.uniform(0, 20, (P // 2, d_in)), rng.normal(0, 0.2, (P - P // 2, d_in))])
+    G = torch.from_numpy(G.astype(np.float32)).to(gpu)
+    net = nepx.EnergyNet.random(d_in, widths, seed=5, act=act, en_min=-2.0, en_max=3.0, device=gpu,
+                                precision="f32")
+    n_dev = torch.tensor([P], dtype=torch.int32, device=gpu)
+    cn = net.cstruct()
+    out = []
+    for maxp in (P, 80000):   # 16 tiles: one per block; 5,000 tiles: one per wave
+        E = torch.full((P,), float("nan"), device=gpu)
+        dG = torch.full((P, d_in), float("nan"), device=gpu)
+        _lib.check(_lib.lib.nepx_ffnn_fwd_bwd(C.byref(cn), _lib.ptr(G), _lib.ptr(n_dev), maxp, _lib.ptr(E),
+                                              _lib.ptr(dG), _lib.stream_handle(None)), "nepx_ffnn_fwd_bwd")
+        torch.cuda.synchronize()
+        out.append((E.cpu().numpy(), dG.cpu().numpy()))
+    (E0, g0), (E1, g1) = out
+    assert np.all(np.isfinite(E0)) and np.all(np.isfinite(g0))
+    assert np.array_equal(E0, E1) and np.array_equal(g0, g1)
+    y, g = O.ffnn_forward_backward(G.cpu().numpy().astype(np.float64), net.layers_f64(), act)
+    assert normwise_rel(E0, -2.0 + 5.0 * y) < 1e-5 and normwise_rel(g0, 5.0 * g) < 1e-4
+
+
 def test_ffnn_split_large_inputs_and_energy_range(gpu):
     """The fp16 hi/lo path with descriptor entries up to 4000 (|G| <= Nd^2 bounds real rows far
     below fp16's 65504) and an energy range of 5000: the backward deltas are carried without the
