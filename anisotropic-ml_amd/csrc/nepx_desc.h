@@ -113,6 +113,13 @@ struct DescCfg {
     static constexpr bool ONE_PASS = pass_end(NANG, LMAX, 1) >= LMAX;
 };
 
+// Number of passes over Cfg's l range.
+constexpr int n_passes(int nang, int lmax) {
+    int n = 1;
+    for (int l0 = 1; pass_end(nang, lmax, l0) < lmax; l0 = pass_end(nang, lmax, l0) + 1) ++n;
+    return n;
+}
+
 // A contiguous range of l: accumulator slots lm_slot(l, m, part) - OFF.
 template <int L0_, int L1_>
 struct LRange {

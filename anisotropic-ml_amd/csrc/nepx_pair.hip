@@ -85,6 +85,107 @@ __global__ __launch_bounds__(kBlock) NEPX_ADJ_ATTR void k_adjoint(nepx_pair_mode
     }
 }
 
+// Multi-pass shapes with few pairs: the l-range passes of a pair run on different waves of a block
+// (four waves, one per SIMD, as the register-heavy passes need; 64 pairs per block).  Wave w takes
+// passes w and NP - 1 - w (the cheap low-l passes with the dear high-l ones).  The forward writes
+// each pass's descriptor entries as k_desc_fwd does; the adjoint leaves each pass's per-body sums
+// in LDS and sums them in pass order, so both give k_desc_fwd's and k_adjoint's results bit for
+// bit (every pass's sums start from zero, as k_adjoint's running sums do not become -0).
+template <class Cfg>
+constexpr int pass_waves() {
+    constexpr int np = n_passes(Cfg::NANG, Cfg::LMAX);
+    return np < 4 ? np : 4;
+}
+template <class Cfg>
+__device__ __forceinline__ int pass_owner(int p) {
+    constexpr int np = n_passes(Cfg::NANG, Cfg::LMAX), w = pass_waves<Cfg>();
+    return p < w ? p : (np - 1 - p < w ? np - 1 - p : p % w);
+}
+
+template <class Cfg>
+__global__ __launch_bounds__(64 * pass_waves<Cfg>()) NEPX_ADJ_ATTR void k_desc_fwd_passes(
+    nepx_pair_model m, const float4* __restrict__ qf, const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+    const float4* __restrict__ pt, const int32_t* __restrict__ np_dev, int64_t maxp, float* __restrict__ G) {
+    const int P = n_pairs(np_dev, maxp);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int idx = blockIdx.x * 64 + (threadIdx.x & 63); idx < P; idx += gridDim.x * 64) {
+        float4 qi, qj, t;
+        load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+        const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
+        const PairSites<Cfg> sites{m, g, 1.f / m.r_nep};
+        float* row = G + (size_t)idx * Cfg::D;
+        int k = 0;
+        for_passes<Cfg>([&](auto range, auto first) {
+            if (pass_owner<Cfg>(k++) == wave) {
+                using LR = decltype(range);
+                float grad[Cfg::NRAD + 1];
+                float A[Cfg::NANG + 1][LR::N];
+                accumulate_sites<Cfg, LR, decltype(first)::value>(sites, grad, A);
+                desc_finish<Cfg, LR>(A, row);
+                if constexpr (decltype(first)::value) {
+#pragma unroll
+                    for (int n = 0; n <= Cfg::NRAD; ++n) row[n] = grad[n];
+                }
+            }
+        });
+    }
+}
+
+template <class Cfg>
+__global__ __launch_bounds__(64 * pass_waves<Cfg>()) NEPX_ADJ_ATTR void k_adjoint_passes(
+    nepx_pair_model m, const float4* __restrict__ qf, const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+    const float4* __restrict__ pt, const int32_t* __restrict__ np_dev, int64_t maxp,
+    const float* __restrict__ dEdG, const float* __restrict__ E, float clip, float* __restrict__ out) {
+    constexpr int NP = n_passes(Cfg::NANG, Cfg::LMAX);
+    __shared__ float part[NP][12][64];  // per pass: G0, T0, G1, T1 (xyz) of the block's 64 pairs
+    const int P = n_pairs(np_dev, maxp);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int base = blockIdx.x * 64; base < P; base += gridDim.x * 64) {  // uniform over the block
+        const int idx = base + lane;
+        const bool valid = idx < P;
+        AdjAcc arms;  // pass 0 sets the pair's arm sums
+        if (valid) {
+            float4 qi, qj, t;
+            load_pair(qf, pi, pj, pt, idx, qi, qj, t);
+            const PairIn g = pair_in<Cfg>(m, qi, qj, t.x, t.y, t.z);
+            const float(&dG)[Cfg::D] = *reinterpret_cast<const float(*)[Cfg::D]>(dEdG + (size_t)idx * Cfg::D);
+            int k = 0;
+            for_passes<Cfg>([&](auto range, auto first) {
+                const int p = k++;
+                if (pass_owner<Cfg>(p) == wave) {
+                    using LR = decltype(range);
+                    float grad[Cfg::NRAD + 1];
+                    float A[Cfg::NANG + 1][LR::N];
+                    AdjAcc a;
+                    desc_accumulate<Cfg, LR, false>(m, g, grad, A);
+                    desc_adjoint<Cfg, LR, decltype(first)::value>(m, g, dG, A, a);
+                    const float v[12] = {a.G0x, a.G0y, a.G0z, a.T0x, a.T0y, a.T0z,
+                                         a.G1x, a.G1y, a.G1z, a.T1x, a.T1y, a.T1z};
+#pragma unroll
+                    for (int c = 0; c < 12; ++c) part[p][c][lane] = v[c];
+                    if constexpr (decltype(first)::value) arms = a;
+                }
+            });
+        }
+        __syncthreads();
+        if (wave == 0 && valid) {  // pass 0 runs on wave 0, so `arms` is set here
+            AdjAcc acc;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                acc.G0x += part[p][0][lane]; acc.G0y += part[p][1][lane]; acc.G0z += part[p][2][lane];
+                acc.T0x += part[p][3][lane]; acc.T0y += part[p][4][lane]; acc.T0z += part[p][5][lane];
+                acc.G1x += part[p][6][lane]; acc.G1y += part[p][7][lane]; acc.G1z += part[p][8][lane];
+                acc.T1x += part[p][9][lane]; acc.T1y += part[p][10][lane]; acc.T1z += part[p][11][lane];
+            }
+            acc.A0x = arms.A0x; acc.A0y = arms.A0y; acc.A0z = arms.A0z;
+            acc.A1x = arms.A1x; acc.A1y = arms.A1y; acc.A1z = arms.A1z;
+            store_out(out, idx, adj_finish<Cfg>(acc), E ? E[idx] : 0.f, clip);
+        }
+        __syncthreads();  // the sums are read before the next pairs' passes write them
+    }
+}
+
 template <class N, int NT, int PREC, int ACT>
 __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, const void* __restrict__ Whg,
                                                  float en_min, float scale, int D, const float* __restrict__ G,
@@ -119,49 +220,65 @@ __global__ __launch_bounds__(kBlock) void k_ffnn(const float* __restrict__ Wg, c
     }
 }
 
-// FP32 network for few pairs: one 16-pair tile per block, and the MT waves of the block take one
-// 16-unit block of every layer each (wave w computes units 16w..16w+15 of every hidden layer and
-// of every backward delta, over all k-steps), so that a tile's chain of layers is spread over MT
-// waves instead of running on one.  The layer outputs go through LDS to the next layer's B
-// operands.  Every accumulator takes the same products in the same order as ffnn_tiles, and the
-// output sum is formed the same way, so E and dE/dG equal k_ffnn's bit for bit.
+// FP32 network for few pairs: one 16-pair tile per block, and the block's waves take the 16-unit
+// blocks of every layer between them (wave w computes blocks MPW w .. MPW w + MPW - 1 of every
+// hidden layer and of every backward delta, over all k-steps), so that a tile's chain of layers is
+// spread over MT / MPW waves instead of running on one.  The layer outputs go through LDS to the
+// next layer's B operands.  Every accumulator takes the same products in the same order as
+// ffnn_tiles, and the output sum is formed the same way, so E and dE/dG equal k_ffnn's bit for
+// bit.  One block per wave (MPW = 1): the 640-thread block of a 160-wide network needs 131 VGPRs
+// per lane, within the 168 that three waves per SIMD leave; each layer's weight reads are kept in
+// that layer, or the compiler hoists them all and spills.
+template <class N>
+constexpr int coop_mpw() { return 1; }
+
 template <class N, int ACT>
-__global__ __launch_bounds__(N::MT * 64) void k_ffnn_coop(const float* __restrict__ W, float en_min, float scale,
-                                                         int D, const float* __restrict__ G,
-                                                         const int32_t* __restrict__ np_dev, int64_t maxp,
-                                                         float* __restrict__ E, float* __restrict__ dEdG) {
+__global__ __launch_bounds__(N::MT / coop_mpw<N>() * 64) void k_ffnn_coop(
+    const float* __restrict__ W, float en_min, float scale, int D, const float* __restrict__ G,
+    const int32_t* __restrict__ np_dev, int64_t maxp, float* __restrict__ E, float* __restrict__ dEdG) {
     constexpr NetLayout Lay = N::layout();
-    constexpr int MT = N::MT;
+    constexpr int MT = N::MT, MPW = coop_mpw<N>();
+    static_assert(MT % MPW == 0, "unit blocks split evenly over the waves");
     __shared__ v4f xs[2][MT][64];  // one layer's outputs: block c of lane l at [c][l]
     const int P = n_pairs(np_dev, maxp);
     int lane = threadIdx.x & 63;
     asm volatile("" : "+v"(lane));  // keeps the weight reads inside the tile loop
     const int g = lane >> 4;
-    const int mt = threadIdx.x >> 6;  // this wave's 16-unit block
+    const int m0 = (threadIdx.x >> 6) * MPW;  // this wave's first 16-unit block
     const int nt = (P + 15) / 16;
     for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int p = tile * 16 + (lane & 15);
         v4f gin[N::MD];
         gather_chunks<N::MD>(G + (size_t)p * D, p < P, D, gin);
-        v4f hown[N::NL];  // this wave's block of every hidden layer
+        v4f hown[N::NL][MPW];  // this wave's blocks of every hidden layer
         int buf = 0;
 #pragma unroll
         for (int k = 0; k < N::NL; ++k) {
+            asm volatile("" : "+v"(lane));  // each layer's weight reads stay in that layer
             const float* Wf = W + Lay.fwd_off(k);
             const int ks = (k == 0) ? N::KD : N::KH;
-            const float4 b4 = *reinterpret_cast<const float4*>(W + Lay.bias_off(k) + 16 * mt + 4 * g);
-            v4f acc = v4f{b4.x, b4.y, b4.z, b4.w};
+            v4f acc[MPW];
+#pragma unroll
+            for (int i = 0; i < MPW; ++i) {
+                const float4 b4 = *reinterpret_cast<const float4*>(W + Lay.bias_off(k) + 16 * (m0 + i) + 4 * g);
+                acc[i] = v4f{b4.x, b4.y, b4.z, b4.w};
+            }
 #pragma unroll
             for (int c = 0; c < ks / 4; ++c) {
                 const v4f x = (k == 0) ? gin[c < N::MD ? c : 0] : xs[buf ^ 1][c][lane];
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Wf[(mt * ks + 4 * c + r) * 64 + lane], x[r], acc, 0,
-                                                              0, 0);
+#pragma unroll
+                    for (int i = 0; i < MPW; ++i)
+                        acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(Wf[((m0 + i) * ks + 4 * c + r) * 64 + lane],
+                                                                     x[r], acc[i], 0, 0, 0);
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) hown[k][r] = act_fwd(acc[r], ACT);
-            xs[buf][mt][lane] = hown[k];
+            for (int i = 0; i < MPW; ++i) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hown[k][i][r] = act_fwd(acc[i][r], ACT);
+                xs[buf][m0 + i][lane] = hown[k][i];
+            }
             __syncthreads();
             buf ^= 1;
         }
@@ -179,39 +296,51 @@ __global__ __launch_bounds__(N::MT * 64) void k_ffnn_coop(const float* __restric
         y += __shfl_xor(y, 16);
         y += __shfl_xor(y, 32);
         y += W[Lay.bout_off()];
-        if (mt == 0 && p < P && lane < 16) E[p] = en_min + scale * y;
-        v4f delta;
-        {
-            const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * mt + 4 * g);
+        if (m0 == 0 && p < P && lane < 16) E[p] = en_min + scale * y;
+        v4f delta[MPW];
+#pragma unroll
+        for (int i = 0; i < MPW; ++i) {
+            const float4 w4 = *reinterpret_cast<const float4*>(wo + 16 * (m0 + i) + 4 * g);
             const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-            for (int r = 0; r < 4; ++r) delta[r] = scale * wv[r] * act_der(hown[N::NL - 1][r], ACT);
+            for (int r = 0; r < 4; ++r) delta[i][r] = scale * wv[r] * act_der(hown[N::NL - 1][i][r], ACT);
         }
-        // backward: layer k's deltas through LDS, then this wave's block of layer k-1 (k = 0: dE/dG)
+        // backward: layer k's deltas through LDS, then this wave's blocks of layer k-1 (k = 0: dE/dG)
 #pragma unroll
         for (int k = N::NL - 1; k >= 0; --k) {
-            xs[buf][mt][lane] = delta;
+#pragma unroll
+            for (int i = 0; i < MPW; ++i) xs[buf][m0 + i][lane] = delta[i];
             __syncthreads();
             buf ^= 1;
-            if (k == 0 && mt >= N::MD) break;
+            if (k == 0 && m0 >= N::MD) break;
+            asm volatile("" : "+v"(lane));
             const float* Wb = W + Lay.bwd_off(k);
-            v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+            const int mtn = (k == 0) ? N::MD : MT;  // output blocks of this product
+            v4f acc[MPW];
+#pragma unroll
+            for (int i = 0; i < MPW; ++i) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int c = 0; c < N::KH / 4; ++c) {
                 const v4f x = xs[buf ^ 1][c][lane];
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Wb[(mt * N::KH + 4 * c + r) * 64 + lane], x[r], acc,
-                                                              0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < MPW; ++i)
+                        if (m0 + i < mtn)
+                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                Wb[((m0 + i) * N::KH + 4 * c + r) * 64 + lane], x[r], acc[i], 0, 0, 0);
             }
-            if (k > 0) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) delta[r] = acc[r] * act_der(hown[k - 1][r], ACT);
-            } else {
+            for (int i = 0; i < MPW; ++i) {
+                if (k > 0) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int e = 16 * mt + 4 * g + r;
-                    if (p < P && e < D) dEdG[(size_t)p * D + e] = acc[r];
+                    for (int r = 0; r < 4; ++r) delta[i][r] = acc[i][r] * act_der(hown[k - 1][i][r], ACT);
+                } else if (m0 + i < N::MD) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int e = 16 * (m0 + i) + 4 * g + r;
+                        if (p < P && e < D) dEdG[(size_t)p * D + e] = acc[i][r];
+                    }
                 }
             }
         }
@@ -409,6 +538,45 @@ static int grid_for(int64_t maxp, int per_block) {
     X(6, 10, 10, 10, CfgCubeV2L10)                  \
     X(4, 10, 10, 10, CfgThV1L10)
 
+// Multi-pass shapes with at most two blocks of 64 pairs per CU run their passes on separate waves
+// (k_desc_fwd_passes, k_adjoint_passes; same results): one lane per pair would leave most SIMDs idle.
+template <class Cfg>
+static bool by_passes(int64_t maxp) {
+    return !Cfg::ONE_PASS && (maxp + 63) / 64 <= 2 * (int64_t)device_cus();
+}
+static int pass_grid(int64_t maxp) { return (int)((maxp + 63) / 64); }
+
+template <class T>
+static void launch_desc_fwd(const nepx_pair_model& m, const float* qf, const int32_t* pair_i, const int32_t* pair_j,
+                            const float* pair_t, const int32_t* np_dev, int64_t maxp, float* G, void* stream) {
+    if constexpr (!T::ONE_PASS) {
+        if (by_passes<T>(maxp)) {
+            hipLaunchKernelGGL(k_desc_fwd_passes<T>, dim3(pass_grid(maxp)), dim3(64 * pass_waves<T>()), 0,
+                               as_stream(stream), m, (const float4*)qf, pair_i, pair_j, (const float4*)pair_t,
+                               np_dev, maxp, G);
+            return;
+        }
+    }
+    hipLaunchKernelGGL(k_desc_fwd<T>, dim3(grid_for(maxp, kBlock)), dim3(kBlock), 0, as_stream(stream), m,
+                       (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, G);
+}
+
+template <class T>
+static void launch_adjoint(const nepx_pair_model& m, const float* qf, const int32_t* pair_i, const int32_t* pair_j,
+                           const float* pair_t, const int32_t* np_dev, int64_t maxp, const float* dEdG,
+                           const float* E, float clip, float* out, void* stream) {
+    if constexpr (!T::ONE_PASS) {
+        if (by_passes<T>(maxp)) {
+            hipLaunchKernelGGL(k_adjoint_passes<T>, dim3(pass_grid(maxp)), dim3(64 * pass_waves<T>()), 0,
+                               as_stream(stream), m, (const float4*)qf, pair_i, pair_j, (const float4*)pair_t,
+                               np_dev, maxp, dEdG, E, clip, out);
+            return;
+        }
+    }
+    hipLaunchKernelGGL(k_adjoint<T>, dim3(grid_for(maxp, kBlock)), dim3(kBlock), 0, as_stream(stream), m,
+                       (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, dEdG, E, clip, out);
+}
+
 static bool desc_supported(const nepx_pair_model* m) {
 #define X(ns, a, b, c, T) \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c) return true;
@@ -582,11 +750,9 @@ extern "C" int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, co
     if (rc) return rc;
     NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && G, NEPX_EINVAL, "null buffer");
     if (maxp <= 0) return NEPX_OK;
-    const int grid = grid_for(maxp, kBlock);
 #define X(ns, a, b, c, T)                                                                           \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c) {                        \
-        hipLaunchKernelGGL(k_desc_fwd<T>, dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,       \
-                           (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, G); \
+        launch_desc_fwd<T>(*m, qf, pair_i, pair_j, pair_t, np_dev, maxp, G, stream);                \
         NEPX_LAUNCH_CHECK();                                                                        \
         return NEPX_OK;                                                                             \
     }
@@ -663,7 +829,7 @@ static bool launch_ffnn_coop(const nepx_net* net, float scale, int d, const floa
     const bool f32 = net->precision == NEPX_NET_F32 || T::HP % 32 != 0;
     const int64_t tiles = (maxp + 15) / 16;
     if (!f32 || tiles > 4 * (int64_t)device_cus()) return false;
-    const dim3 grid((unsigned)tiles), block(T::MT * 64);
+    const dim3 grid((unsigned)tiles), block(T::MT / coop_mpw<T>() * 64);
     if (net->act == NEPX_ACT_TANH)
         hipLaunchKernelGGL((k_ffnn_coop<T, NEPX_ACT_TANH>), grid, block, 0, as_stream(stream), net->w_packed,
                            net->en_min, scale, d, G, np_dev, maxp, E, dEdG);
@@ -706,12 +872,9 @@ extern "C" int nepx_descriptor_adjoint(const nepx_pair_model* m, const float* qf
     if (rc) return rc;
     NEPX_REQUIRE(qf && pair_i && pair_j && pair_t && np_dev && dEdG && out, NEPX_EINVAL, "null buffer");
     if (maxp <= 0) return NEPX_OK;
-    const int grid = grid_for(maxp, kBlock);
 #define X(ns, a, b, c, T)                                                                            \
     if (m->n_sites == ns && m->nrad == a && m->nang == b && m->lmax == c) {                         \
-        hipLaunchKernelGGL(k_adjoint<T>, dim3(grid), dim3(kBlock), 0, as_stream(stream), *m,         \
-                           (const float4*)qf, pair_i, pair_j, (const float4*)pair_t, np_dev, maxp, dEdG, \
-                           E, clip, out);                                                            \
+        launch_adjoint<T>(*m, qf, pair_i, pair_j, pair_t, np_dev, maxp, dEdG, E, clip, out, stream);  \
         NEPX_LAUNCH_CHECK();                                                                         \
         return NEPX_OK;                                                                              \
     }

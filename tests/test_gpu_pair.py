@@ -170,6 +170,38 @@ def test_adjoint_matches_oracle(gpu, shape_name, hyp):
     assert max(eF, eTi, eTj) < 1e-4
 
 
+@pytest.mark.parametrize("shape_name,hyp", CASES)
+def test_pass_kernels_equal_pair_kernels(gpu, shape_name, hyp):
+    """Few pairs of a multi-pass shape run their l-range passes on separate waves (descriptor and
+    adjoint); the results equal the one-lane-per-pair kernels' bit for bit.  The capacity argument
+    picks the kernels (400 pairs either way; one-pass shapes take the same kernels twice)."""
+    import ctypes as C
+    import nepx
+    from nepx import _lib
+    shape, (q, i, j, t), T = _inputs(gpu, shape_name, n_pairs=400, seed=6)
+    model = nepx.make_pair_model(shape, hyp)
+    D = nepx.n_descriptor(*hyp)
+    P = len(i)
+    n_dev = torch.tensor([P], dtype=torch.int32, device=gpu)
+    dG = torch.from_numpy(np.random.default_rng(13).standard_normal((P, D)).astype(np.float32)).to(gpu)
+    E = torch.from_numpy(np.linspace(-1, 1, P).astype(np.float32)).to(gpu)
+    args = (_lib.ptr(T["qf"]), _lib.ptr(T["pi"]), _lib.ptr(T["pj"]), _lib.ptr(T["pt"]), _lib.ptr(n_dev))
+    res = []
+    for maxp in (P, 100000):   # 7 blocks of 64 pairs with the passes on separate waves; one lane per pair
+        G = torch.full((P, D), float("nan"), device=gpu)
+        out = torch.full((P, 12), float("nan"), device=gpu)
+        s = _lib.stream_handle(None)
+        _lib.check(_lib.lib.nepx_descriptor_fwd(C.byref(model), *args, maxp, _lib.ptr(G), s), "nepx_descriptor_fwd")
+        _lib.check(_lib.lib.nepx_descriptor_adjoint(C.byref(model), *args, maxp, _lib.ptr(dG), _lib.ptr(E), 100.0,
+                                                    _lib.ptr(out), s), "nepx_descriptor_adjoint")
+        torch.cuda.synchronize()
+        res.append((G.cpu().numpy(), out.cpu().numpy()))
+    (G0, o0), (G1, o1) = res
+    assert np.all(np.isfinite(G0)) and np.all(np.isfinite(o0))
+    assert np.array_equal(G0, G1)
+    assert np.array_equal(o0, o1)
+
+
 @pytest.fixture(params=[4, 1], ids=["lanes4", "lanes1"])
 def pair_lanes(request):
     """Fused kernel variant: four lanes per pair (site sums on MFMA) or one lane per pair."""
