@@ -266,6 +266,58 @@ __device__ __forceinline__ void radial_derivs(float rho, Fn&& fn) {
     }
 }
 
+// Radial part of the adjoint at one site: f_n and f_n' for n < NA (the operands of the c and d
+// products) and the return value sum_n w_n f_n'(rho) over n = 0..NMAX, w_n = dE/dg_rad[n].  The
+// derivative is taken through T_n' = n U_{n-1} (U of the second kind), and the terms n >= NA, which
+// enter only the sum, are gathered as hdx sum (n w_n) U_{n-1} + hdf (sum w_n T_n + sum w_n): four
+// instructions each instead of seven.  nw[n] = n w_n and w_hi = sum_{n >= NA} w_n are per pair.
+template <int NMAX, int NA>
+__device__ __forceinline__ float radial_adjoint(float rho, const float (&w)[NMAX + 1], const float (&nw)[NMAX + 1],
+                                                float w_hi, float (&fa)[NA], float (&da)[NA]) {
+    const float xm = rho - 1.f;
+    const float x = 2.f * xm * xm - 1.f;
+    const float sp = sin_pi(rho), cp = cos_pi(rho);
+    const float hf = 0.25f * (1.f + cp);                // fc / 2
+    const float hdx = 2.f * xm * (0.5f * (1.f + cp));    // (dx/drho) fc / 2
+    const float hdf = -0.25f * (float)M_PI * sp;         // fc' / 2
+    const float x2 = 2.f * x;
+    float t0 = 1.f, t1 = x;   // T_{n-2}, T_{n-1}
+    float u0 = 1.f, u1 = x2;  // U_{n-3}, U_{n-2} (from n = 3)
+    float s_lo = 0.f, s_t = 0.f, s_u = 0.f;
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+        float tn, un;  // T_n, U_{n-1}
+        if (n == 0) {
+            tn = 1.f;
+            un = 0.f;
+        } else if (n == 1) {
+            tn = x;
+            un = 1.f;
+        } else {
+            tn = x2 * t1 - t0;
+            t0 = t1;
+            t1 = tn;
+            if (n == 2) {
+                un = x2;
+            } else {
+                un = x2 * u1 - u0;
+                u0 = u1;
+                u1 = un;
+            }
+        }
+        if (n < NA) {
+            const float dn = (float)n * un;
+            fa[n] = tn * hf + hf;
+            da[n] = dn * hdx + (tn * hdf + hdf);
+            s_lo += w[n] * da[n];
+        } else {
+            s_t += w[n] * tn;
+            s_u += nw[n] * un;
+        }
+    }
+    return s_lo + (hdx * s_u + hdf * (s_t + w_hi));
+}
+
 // Site visitors.  visit(site, body_done) calls site(s, w) for every site
 // (geometry relative to the cloud centroid, per-site weight w) and
 // body_done(b) after the sites of body b.
@@ -408,7 +460,7 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
     float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f;
 #pragma unroll
     for (int mm = 0; mm <= LR::L1; ++mm) {
-        float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f};
+        float U[2] = {0.f, 0.f}, V[2] = {0.f, 0.f}, W[2] = {0.f, 0.f}, Dq[2] = {0.f, 0.f};
 #pragma unroll
         for (int l = (mm > LR::L0 ? mm : LR::L0); l <= LR::L1; ++l) {
             const float ql = h.q(l, mm);
@@ -417,15 +469,15 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
 #pragma unroll
             for (int part = 0; part < (mm ? 2 : 1); ++part) {
                 const int o = lm_slot(l, mm, part) - LR::OFF;
-                const float z = mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]);
                 float c, d;
                 cd(o, c, d);
-                dEdrho += d * z;
+                Dq[part] += d * ql;  // sum_l d_lm Q_l^m; times C_m or S_m below
                 if (mm + 1 <= l - 1) U[part] += qa * c;
                 if (mm <= l - 1) V[part] += qc * c;
                 if (mm >= 1) W[part] += ql * c;
             }
         }
+        dEdrho += mm == 0 ? Dq[0] : h.C[mm] * Dq[0] + h.S[mm] * Dq[1];
         if (mm == 0) {
             E1 += U[0];
             Gz += V[0];

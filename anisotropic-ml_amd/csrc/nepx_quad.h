@@ -254,9 +254,14 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
         }
         wave_lds_order();
         // ---- adjoint ----
-        float dGr[Cfg::NRAD + 1];
+        float dGr[Cfg::NRAD + 1], ndGr[Cfg::NRAD + 1];
+        float dGr_hi = 0.f;
 #pragma unroll
-        for (int n = 0; n <= Cfg::NRAD; ++n) dGr[n] = grow[n];
+        for (int n = 0; n <= Cfg::NRAD; ++n) {
+            dGr[n] = grow[n];
+            ndGr[n] = (float)n * dGr[n];
+            if (n >= NA) dGr_hi += dGr[n];
+        }
         const float E = epair;
         // B = dE/dA = 2 K^2 dE/dg_ang[n][l] A, in place (rows n >= NA stay zero)
 #pragma unroll
@@ -277,14 +282,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             bool bj;
             const SiteGeo sg = site(s, bj);
             float fa[NA], da[NA];
-            float dEdrho = 0.f;
-            radial_derivs<Cfg::NRAD>(sg.rho, [&](int n, float f, float df) {
-                dEdrho += dGr[n] * df;
-                if (n < NA) {
-                    fa[n] = f;
-                    da[n] = df;
-                }
-            });
+            float dEdrho = radial_adjoint<Cfg::NRAD, NA>(sg.rho, dGr, ndGr, dGr_hi, fa, da);
             // the c/d products issue as one block, with no VALU work scheduled between them, while
             // the other wave on the SIMD issues its VALU work (4 % faster at tet10 and cyl12 than
             // letting the compiler interleave them).  The wave drops to the lower priority for the
