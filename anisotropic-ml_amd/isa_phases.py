@@ -2,17 +2,18 @@
 
 Compiles csrc/nepx_quad.hip for gfx950 to device assembly (same flags as build.py), picks the
 k_fused_quad instance of the given descriptor shape (number of sites per body, hypers), network
-precision and activation, cuts it into basic blocks and groups them by their place in the
-kernel's main loop (one iteration = 16 pairs per wave):
+precision and activation, and counts the instructions of one iteration of the kernel's main loop
+(16 pairs per wave).  Both site loops are unrolled, so the iteration is cut at the `s_setprio 1`
+that closes each site trip's block of 4x4x1 matrix products:
 
-  setup     blocks from the loop head to the forward site loop (pair loads, rotation matrices,
-            pair geometry)
-  forward   the forward site loop (one site per lane per trip; trips = sites per lane)
-  network   the blocks between the two site loops that issue the network's 16x16x32 MFMAs (with
-            whatever the compiler schedules into the same basic block, e.g. the descriptor epilogue)
-  between   the other blocks between the site loops (adjoint prelude, descriptor epilogue if apart)
-  adjoint   the adjoint site loop (same trip count)
-  tail      the rest of the iteration (per-pair sums, clip, stores)
+  forward   the trips of the forward site loop (site geometry, radial values, harmonics, LDS
+            exchange, products); the first trip also holds the iteration's setup (pair loads,
+            rotation matrices, pair geometry)
+  network   from the last forward block to the first adjoint block: descriptor epilogue, the
+            network's 16x16x32 products, adjoint prelude, and the first adjoint trip's radial part
+  adjoint   the later adjoint trips (each: the previous trip's gradient terms and sums, this
+            trip's geometry and radial part, its c/d products)
+  tail      after the last adjoint block: its gradient terms, per-pair sums, clip, stores
 
 Usage: python anisotropic-ml_amd/isa_phases.py [--sites 10] [--hypers 11 6 4] [--prec 1] [--act 0]
 """
@@ -45,25 +46,6 @@ def kernel_lines(lines, sites, hyp, prec, act):
     return lines[start:end]
 
 
-def blocks_of(body):
-    blocks, cur = [], ["entry", []]
-    for l in body[1:]:
-        m = re.match(r"^(\.LBB\S+):", l)
-        if m:
-            blocks.append(cur)
-            cur = [m.group(1), []]
-            continue
-        t = l.strip()
-        if not t or t.startswith(";") or t.startswith("."):
-            continue
-        cur[1].append(t)
-        if t.startswith(("s_cbranch", "s_branch")):  # a branch ends the block; the fall-through starts one
-            blocks.append(cur)
-            cur = [cur[0] + "+", []]
-    blocks.append(cur)
-    return [b for b in blocks if b[1]]
-
-
 def kind(op):
     if op.startswith("v_mfma"):
         return "mfma"
@@ -74,6 +56,26 @@ def kind(op):
     if op.startswith(("global_", "buffer_")):
         return "vmem"
     return "other"
+
+
+def main_loop(body):
+    """Instructions of the main loop's body, in order: from the target of the last backward branch
+    to that branch."""
+    labels, ins = {}, []
+    for l in body[1:]:
+        m = re.match(r"^(\.LBB\S+):", l)
+        if m:
+            labels[m.group(1)] = len(ins)
+            continue
+        t = l.strip()
+        if t and not t.startswith((";", ".")):
+            ins.append(t)
+    back = [(k, labels[x.split()[-1]]) for k, x in enumerate(ins)
+            if x.startswith(("s_branch", "s_cbranch")) and labels.get(x.split()[-1], len(ins)) < k]
+    if not back:
+        raise SystemExit("no main loop found")
+    end, head = max(back, key=lambda b: b[0] - b[1])
+    return ins[head:end + 1]
 
 
 def main():
@@ -90,53 +92,30 @@ def main():
         lines = open(out).read().split("\n")
     body = kernel_lines(lines, a.sites, a.hypers, a.prec, a.act)
     vgpr = next((l.split()[-1] for l in body if ".amdhsa_next_free_vgpr" in l), "?")
-    blocks = blocks_of(body)
-    # the site loops: blocks that branch back to their own label and issue the 4x4 MFMAs
-    loops = [k for k, (lab, ins) in enumerate(blocks)
-             if ins[-1].startswith("s_cbranch") and ins[-1].split()[-1] == lab and
-             any(x.startswith("v_mfma_f32_4x4x1") for x in ins)]
-    if len(loops) < 2:
-        raise SystemExit(f"expected two site loops, found {len(loops)}")
-    f_loop, a_loop = loops[0], loops[1]
-    index = {lab: k for k, (lab, _) in enumerate(blocks)}
-    # the iteration's loop head: target of the branches after the adjoint loop that jump back above
-    # the forward loop (the last of them ends the tail); the head and the blocks up to the forward
-    # loop are the setup
-    back = [(k, index[ins[-1].split()[-1]]) for k, (lab, ins) in enumerate(blocks)
-            if k > a_loop and ins[-1].startswith(("s_branch", "s_cbranch")) and
-            index.get(ins[-1].split()[-1], len(blocks)) < f_loop]
-    if not back:
-        raise SystemExit("no branch back to the loop head")
-    tail_end, head = back[-1]
-    phases = collections.OrderedDict((p, collections.Counter()) for p in
-                                     ("setup", "forward", "network", "between", "adjoint", "tail"))
-    for k, (lab, ins) in enumerate(blocks):
-        c = collections.Counter(kind(x.split()[0]) for x in ins)
-        if k == f_loop:
-            phases["forward"].update(c)
-        elif k == a_loop:
-            phases["adjoint"].update(c)
-        elif head <= k < f_loop:
-            phases["setup"].update(c)
-        elif f_loop < k < a_loop:
-            net = any(x.startswith("v_mfma_f32_16x16x32") for x in ins)
-            phases["network" if net else "between"].update(c)
-        elif a_loop < k <= tail_end:
-            phases["tail"].update(c)
-        elif k > tail_end and not ins[-1].startswith("s_endpgm"):
-            phases["between"].update(c)  # out-of-line blocks of the region between the loops
+    loop = main_loop(body)
     trips = -(-2 * a.sites // 4)
+    segs, cur = [], []
+    for x in loop:
+        cur.append(x)
+        if x.split()[0] == "s_setprio" and x.split()[-1] == "1":
+            segs.append(cur)
+            cur = []
+    segs.append(cur)
+    if len(segs) != 2 * trips + 1:
+        raise SystemExit(f"expected {2 * trips} site-trip blocks, found {len(segs) - 1}")
+    names = ["forward"] * trips + ["network"] + ["adjoint"] * (trips - 1) + ["tail"]
+    phases = collections.OrderedDict((p, collections.Counter()) for p in ("forward", "network", "adjoint", "tail"))
+    for n, seg in zip(names, segs):
+        phases[n].update(kind(x.split()[0]) for x in seg)
     print(f"k_fused_quad sites/body={a.sites} hypers={tuple(a.hypers)} prec={a.prec} act={a.act}: "
-          f"{vgpr} VGPRs; site-loop trips per iteration = {trips}")
-    print("| phase | VALU | MFMA | LDS | VMEM | per iteration (VALU) |")
+          f"{vgpr} VGPRs; site trips per iteration = {trips} (unrolled)")
+    print("| phase | VALU | MFMA | LDS | VMEM | segments |")
     print("|---|---:|---:|---:|---:|---:|")
     tot = 0
     for p, c in phases.items():
-        mult = trips if p in ("forward", "adjoint") else 1
-        tot += c["valu"] * mult
-        print(f"| {p}{' (per trip)' if mult > 1 else ''} | {c['valu']} | {c['mfma']} | {c['lds']} | {c['vmem']} | "
-              f"{c['valu'] * mult} |")
-    print(f"| total per 16-pair wave iteration | | | | | {tot} |")
+        tot += c["valu"]
+        print(f"| {p} | {c['valu']} | {c['mfma']} | {c['lds']} | {c['vmem']} | {names.count(p)} |")
+    print(f"| total per 16-pair wave iteration | {tot} | | | | |")
 
 
 if __name__ == "__main__":

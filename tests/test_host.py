@@ -593,14 +593,18 @@ def test_poisson_trace_builds_job_arrays_lazily():
     assert j.spec._spec is None and np.array_equal(j.spec.x, x0)
 
 
-def test_isa_phase_counter_finds_both_site_loops():
-    """The static per-phase instruction counter of the four-lane fused kernel (gfx950 assembly)."""
+def test_isa_phase_counter_finds_every_site_trip():
+    """The static per-phase instruction counter of the four-lane fused kernel (gfx950 assembly): the
+    unrolled iteration of the cube8 instance cuts into 4 forward trips, the network and 3 more
+    adjoint trips plus the tail."""
     import subprocess
     import sys
     out = subprocess.run([sys.executable, os.path.join(ROOT, "anisotropic-ml_amd", "isa_phases.py"), "--sites", "8"],
                          capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stderr[-2000:]
-    rows = {l.split("|")[1].strip(): l for l in out.stdout.splitlines() if l.startswith("| ")}
-    assert "forward (per trip)" in rows and "adjoint (per trip)" in rows and "network" in rows
-    total = int(out.stdout.strip().splitlines()[-1].split("|")[-2])
+    rows = {l.split("|")[1].strip(): [c.strip() for c in l.split("|")[2:-1]]
+            for l in out.stdout.splitlines() if l.startswith("| ")}
+    assert rows["forward"][-1] == "4" and rows["network"][-1] == "1" and rows["adjoint"][-1] == "3"
+    assert int(rows["forward"][1]) == 4 * 48 and int(rows["adjoint"][1]) == 3 * 84   # 4x4x1 products per trip
+    total = int(rows["total per 16-pair wave iteration"][0])
     assert 2000 < total < 6000, total
