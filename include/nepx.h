@@ -183,6 +183,9 @@ int nepx_net_pack_lowp(int32_t precision, int32_t d_in, int32_t width, int32_t n
                        uint16_t* host_out);
 
 /* ---- descriptor, energy network, adjoint ------------------------------- */
+/* P = min(*n_pairs_dev, max_pairs) is read on the device; max_pairs sizes the launch.  Up to about
+ * 2 x 64 pairs per CU (descriptor, adjoint of multi-pass shapes) or 4 x 16 per CU (FP32 network)
+ * the work of one pair is spread over several waves; the results are the same bit for bit. */
 /* G (P x D, f32) of compacted pairs.  Replaces pos_to_pts + pts_to_nep_descriptors
  * (DescriptorGenerator.py:163-218,296-432); one row per pair (no FD stencil). */
 int nepx_descriptor_fwd(const nepx_pair_model* m, const float* qf, const int32_t* pair_i,
