@@ -309,10 +309,21 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
                     cc[d] = __builtin_amdgcn_mfma_f32_4x4x1f32(b, fa[n], cc[d], 0, 0, 0);
                     dd[d] = __builtin_amdgcn_mfma_f32_4x4x1f32(b, da[n], dd[d], 0, 0, 0);
                 }
+            Harm<LMAX> h;
+            if constexpr (Q::WAVES_PER_SIMD > 1) {
+                // two waves per SIMD: the harmonics (independent of the products) fill one slot
+                // after every four products of the block (1.4 % faster at tet10; with one wave per
+                // SIMD, cyl12, 2 % slower, so there the block stays pure)
+                h.eval(sg.ux, sg.uy, sg.uz);
+#pragma unroll
+                for (int k = 0; k < 2 * NA * LC / 4; ++k) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);  // four matrix products
+                    __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);  // one VALU instruction
+                }
+            }
             __builtin_amdgcn_sched_barrier(0);
             __builtin_amdgcn_s_setprio(1);
-            Harm<LMAX> h;
-            h.eval(sg.ux, sg.uy, sg.uz);
+            if constexpr (Q::WAVES_PER_SIMD == 1) h.eval(sg.ux, sg.uy, sg.uz);
             float gx, gy, gz;
             angular_site_grad<LRange<1, LMAX>>(
                 h, sg.ux, sg.uy,
