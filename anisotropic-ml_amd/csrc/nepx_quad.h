@@ -288,11 +288,11 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             const SiteGeo sg = site(s, bj);
             float fa[NA], da[NA];
             float dEdrho = radial_adjoint<Cfg::NRAD, NA>(sg.rho, dGr, ndGr, dGr_hi, fa, da);
-            // the c/d products issue as one block, with no VALU work scheduled between them, while
-            // the other wave on the SIMD issues its VALU work (4 % faster at tet10 and cyl12 than
-            // letting the compiler interleave them).  The wave drops to the lower priority for the
-            // block and raises it after, so that the VALU work, the bound, issues first when both
-            // waves of a SIMD are ready (4 % faster at tet10)
+            // the c/d products issue as one block, with no other VALU work than the harmonics below
+            // scheduled between them, while the other wave on the SIMD issues its VALU work (4 %
+            // faster at tet10 and cyl12 than letting the compiler interleave them).  The wave drops
+            // to the lower priority for the block and raises it after, so that the VALU work, the
+            // bound, issues first when both waves of a SIMD are ready (4 % faster at tet10)
             v4f cc[LC], dd[LC];
             __builtin_amdgcn_sched_barrier(0);
             __builtin_amdgcn_s_setprio(0);
