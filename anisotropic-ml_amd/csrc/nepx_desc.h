@@ -453,8 +453,10 @@ __device__ __forceinline__ void desc_finish(const float (&A)[Cfg::NANG + 1][LR::
 //        d/dx (C_m, S_m) = m (C_{m-1}, S_{m-1}),  d/dy (C_m, S_m) = m (-S_{m-1}, C_{m-1}),
 // with the sums over l taken first for each m, so each (l, m) costs a few FMAs.  The radial
 // projection sum_lm l c_lm Z_lm(u_a) is not summed here: by Euler's theorem on the degree-l
-// terms it equals u_a . grad at |u_a| = 1 (site_dEdr).
-template <class LR, class CD>
+// terms it equals u_a . grad at |u_a| = 1 (site_dEdr).  PER_M: sum_lm d_lm Z_lm is formed per m as
+// C_m sum_l d_lm Q_l^m + S_m sum_l ... (fewer multiplies; the four-lane kernel), else per slot (fewer
+// live registers; the one-lane kernels, whose hypers 10 10 10 adjoint would spill more).
+template <class LR, bool PER_M = false, class CD>
 __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float ux, float uy, CD&& cd,
                                                   float& dEdrho, float& gx, float& gy, float& gz) {
     float E1 = 0.f, Gz = 0.f, X1 = 0.f, Y1 = 0.f;
@@ -471,13 +473,17 @@ __device__ __forceinline__ void angular_site_grad(const Harm<LR::L1>& h, float u
                 const int o = lm_slot(l, mm, part) - LR::OFF;
                 float c, d;
                 cd(o, c, d);
-                Dq[part] += d * ql;  // sum_l d_lm Q_l^m; times C_m or S_m below
+                if constexpr (PER_M) {
+                    Dq[part] += d * ql;  // sum_l d_lm Q_l^m; times C_m or S_m below
+                } else {
+                    dEdrho += d * (mm == 0 ? ql : ql * (part ? h.S[mm] : h.C[mm]));
+                }
                 if (mm + 1 <= l - 1) U[part] += qa * c;
                 if (mm <= l - 1) V[part] += qc * c;
                 if (mm >= 1) W[part] += ql * c;
             }
         }
-        dEdrho += mm == 0 ? Dq[0] : h.C[mm] * Dq[0] + h.S[mm] * Dq[1];
+        if constexpr (PER_M) dEdrho += mm == 0 ? Dq[0] : h.C[mm] * Dq[0] + h.S[mm] * Dq[1];
         if (mm == 0) {
             E1 += U[0];
             Gz += V[0];

@@ -325,7 +325,7 @@ __global__ __launch_bounds__(QuadCfg<Cfg>::BLOCK) __attribute__((
             __builtin_amdgcn_s_setprio(1);
             if constexpr (Q::WAVES_PER_SIMD == 1) h.eval(sg.ux, sg.uy, sg.uz);
             float gx, gy, gz;
-            angular_site_grad<LRange<1, LMAX>>(
+            angular_site_grad<LRange<1, LMAX>, true>(
                 h, sg.ux, sg.uy,
                 [&](int o, float& c, float& dv) {
                     c = cc[o / 4][o % 4];
