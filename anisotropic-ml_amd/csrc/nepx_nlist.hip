@@ -414,10 +414,19 @@ __device__ __forceinline__ void eval_slots(const nepx_state& st, const nepx_nlis
         r.i[k] = in[k] ? nl.slot_row[s] : 0;
         r.j[k] = in[k] ? nl.nbr[s] : 0;
     }
+    if (st.n_sys == 1) {  // one system: its particle range stands in for the system-id reads
+        const int p0 = st.sp[0].p_start, p1 = st.sp[0].p_end;
 #pragma unroll
-    for (int k = 0; k < kSlotsPerThread; ++k) {
-        si[k] = st.sys[r.i[k]];
-        sj[k] = st.sys[r.j[k]];
+        for (int k = 0; k < kSlotsPerThread; ++k) {
+            si[k] = r.i[k] >= p0 && r.i[k] < p1 ? 0 : -1;
+            sj[k] = r.j[k] >= p0 && r.j[k] < p1 ? 0 : -1;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kSlotsPerThread; ++k) {
+            si[k] = st.sys[r.i[k]];
+            sj[k] = st.sys[r.j[k]];
+        }
     }
 #pragma unroll
     for (int k = 0; k < kSlotsPerThread; ++k) {
